@@ -1,0 +1,313 @@
+/*
+ * mdsm_c.h -- C ABI of the MI355X-native MDSM dedispersion path
+ * (libmdsm_hip.so, gfx950).
+ *
+ * Plain C types only: plain pointers, sizes and status codes.  Device
+ * buffers are caller-owned device pointers (hipMalloc or any allocator on
+ * the plan's device); `stream` is a hipStream_t passed as void* (NULL =
+ * the null stream).  No call exits the process; every entry point returns
+ * an mdsm_status and mdsm_last_error() holds the message of the most recent
+ * failure on the calling thread.
+ *
+ * Each entry point cites the reference interface it replaces (paths
+ * relative to the reference repository root).
+ */
+#ifndef MDSM_C_H
+#define MDSM_C_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ */
+/* Status, types                                                       */
+/* ------------------------------------------------------------------ */
+
+/* Replaces the reference's fprintf(stderr)+exit(0) error convention
+ * (src/mdsm/dedispersion_thread.cu:11-12,58-59,329-330). */
+typedef enum {
+    MDSM_OK = 0,
+    MDSM_ERR_INVALID = -1,  /* malformed plan, argument or size */
+    MDSM_ERR_NOMEM = -2,    /* host or device allocation failed */
+    MDSM_ERR_DEVICE = -3,   /* HIP runtime error */
+    MDSM_ERR_CAPACITY = -4, /* candidate buffer too small (count still valid) */
+    MDSM_ERR_BUDGET = -5,   /* request can never fit the HBM budget */
+    MDSM_ERR_STATE = -6,    /* call out of order / object closed */
+    MDSM_ERR_PARSE = -7,    /* observation XML could not be parsed */
+    MDSM_ERR_AGAIN = -8     /* nothing ready yet / admission deferred */
+} mdsm_status;
+
+/* Sample storage of the filterbank.  uint8 is the plain byte value
+ * (src/mdsm/file_handler.cpp:255-264); every path accumulates in float32. */
+typedef enum { MDSM_U8 = 0, MDSM_F16 = 1, MDSM_F32 = 2 } mdsm_dtype;
+
+/* Integer shift rule.
+ *   MDSM_SHIFT_REF: shift = (int)(delay_c * dm / tsamp) in float32, with
+ *     delay_c = (float)(4148.741601 * (f_c^-2 - f_top^-2)), f_c = fch1 + foff*c
+ *     formed in float -- bit-exact with src/prototypes/cpu_dedisp.c:52-55,72
+ *     and src/mdsm/dedispersion_manager.cpp:262-265,287-289.
+ *   MDSM_SHIFT_NS: shift = llround(4.148808e3 * dm * (f_c^-2 - f_top^-2) / tsamp)
+ *     in FP64 (physical delay, nearest sample). */
+typedef enum { MDSM_SHIFT_REF = 0, MDSM_SHIFT_NS = 1 } mdsm_shift_rule;
+
+/* Observation of one beam: the subset of SURVEY (src/mdsm/survey.h:13-57)
+ * the dedispersion path reads. */
+typedef struct {
+    uint32_t nchans;   /* C; channel 0 is the highest frequency */
+    float fch1;        /* MHz, channel 0 */
+    float foff;        /* MHz per channel (negative) */
+    float tsamp;       /* seconds */
+    uint32_t nsamp_in; /* S: samples per channel per block, overlap included */
+    int32_t dtype;     /* mdsm_dtype */
+    int32_t shift_rule;/* mdsm_shift_rule */
+} mdsm_obs;
+
+/* One brute-force DM range of a multi-range plan.  With downsample k the
+ * range runs on the k-sample-decimated block at tsamp*k and its output is
+ * the channel sum of k-sample means (exact for uint8 input).  Replaces the
+ * brute-force fields of SURVEY (lowdm, dmstep, tdms; survey.h:23-24) and
+ * the per-pass downsample of SUBBAND_PASSES (survey.h:6-11). */
+typedef struct {
+    float lowdm;
+    float dmstep;
+    uint32_t ndms;
+    uint32_t downsample;
+} mdsm_range;
+
+/* Two-stage subband pass, field for field SUBBAND_PASSES
+ * (src/mdsm/survey.h:6-11; parsed at dedispersion_manager.cpp:158-184). */
+typedef struct {
+    float lowdm, highdm, dmstep, sub_dmstep;
+    uint32_t binsize, ndms, ncalls, calldms;
+} mdsm_pass;
+
+typedef struct mdsm_plan mdsm_plan;
+
+/* ------------------------------------------------------------------ */
+/* Plans                                                               */
+/* ------------------------------------------------------------------ */
+
+/* Build a brute-force multi-range plan on the current HIP device: shift
+ * tables, maxshift, output layout and the device work tables.
+ * Replaces the plan set-up half of initialiseMDSM
+ * (src/mdsm/dedispersion_manager.cpp:286-314) and the shift/maxshift code of
+ * src/prototypes/cpu_dedisp.c:67-76. */
+int mdsm_plan_create(const mdsm_obs *obs, const mdsm_range *ranges,
+                     uint32_t nranges, mdsm_plan **out);
+
+/* Build a two-stage subband plan (nsubs subbands, one or more passes).
+ * Replaces the subband set-up of initialiseMDSM and
+ * src/mdsm/dedispersion_thread.cu:150-210. */
+int mdsm_plan_create_subband(const mdsm_obs *obs, uint32_t nsubs,
+                             const mdsm_pass *passes, uint32_t npasses,
+                             mdsm_plan **out);
+
+void mdsm_plan_destroy(mdsm_plan *plan);
+
+typedef struct {
+    uint32_t nranges;        /* ranges (brute) or passes (subband) */
+    uint32_t ndms_total;     /* rows of the packed output */
+    uint32_t maxshift;       /* full-rate overlap samples the plan needs */
+    uint32_t nchans;
+    uint64_t out_elems;      /* floats in the packed output */
+    uint64_t work_bytes;     /* device workspace (decimated copies, stage 1) */
+    double dm_chan_samples;  /* sum_r D_r*C*N_r (brute-force equivalent) */
+    double adds;             /* float adds the plan performs */
+    double bytes;            /* algorithmic HBM bytes per block */
+    uint32_t is_subband;
+    uint32_t nblocks;        /* DM blocks in the device work table */
+} mdsm_plan_info;
+
+int mdsm_plan_get_info(const mdsm_plan *plan, mdsm_plan_info *info);
+
+typedef struct {
+    float lowdm, dmstep;
+    uint32_t ndms, downsample;
+    uint32_t nout;        /* output samples per DM row */
+    uint32_t maxshift;    /* in decimated samples */
+    uint64_t out_offset;  /* float offset of row 0 in the packed output */
+    uint32_t first_dm;    /* global row index of row 0 */
+    uint32_t ncalls;      /* subband: calls of the pass (stage-1 table rows); brute: ndms */
+} mdsm_range_info;
+
+int mdsm_plan_get_range(const mdsm_plan *plan, uint32_t r, mdsm_range_info *info);
+
+/* Host copy of range r's integer shift table [ndms][nchans] (brute force),
+ * or of pass r's stage-1 table [ncalls][nchans] (subband). */
+int mdsm_plan_shift_table(const mdsm_plan *plan, uint32_t r, int32_t *host_out);
+
+/* Subband pass r's stage-2 table [ncalls*calldms][nsubs]. */
+int mdsm_plan_fine_table(const mdsm_plan *plan, uint32_t r, int32_t *host_out);
+
+/* DM value of every output row (float32), as the reference labels them
+ * (src/pelican-mdsm/modules/MdsmModule.cpp:189,202-210). */
+int mdsm_plan_dm_values(const mdsm_plan *plan, float *host_out);
+
+/* ------------------------------------------------------------------ */
+/* Dedispersion                                                        */
+/* ------------------------------------------------------------------ */
+
+/* Dedisperse one block.  d_in: channel-major [nchans][in_stride] samples of
+ * obs.dtype (in_stride >= nsamp_in).  d_out: out_elems floats, ranges packed
+ * row-major ([ndms_r][nout_r] each, at out_offset_r).  d_work: work_bytes of
+ * device memory, or NULL to use a workspace the plan allocates once.
+ * Replaces the kernel launches of src/mdsm/dedispersion_thread.cu:64-243
+ * (opt_dedisperse_loop, global_for_time_dedisperse_loop, the binning kernels
+ * and opt_dedisperse_subband). */
+int mdsm_dedisperse(mdsm_plan *plan, const void *d_in, int64_t in_stride,
+                    float *d_out, void *d_work, void *stream);
+
+/* k-sample time decimation (sum) of a channel-major block:
+ * d_out[c][t] = sum_{j<k} d_in[c][k t + j], t < nsamp/k.  uint8 input gives
+ * uint16 output, float16/float32 input gives float32.
+ * Replaces binning_kernel/inplace_binning_kernel
+ * (src/mdsm/dedispersion_kernel.cu:203-246). */
+int mdsm_decimate(const void *d_in, int32_t dtype, uint32_t nchans,
+                  int64_t in_stride, uint32_t nsamp, uint32_t k, void *d_out,
+                  int64_t out_stride, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* RFI clipper and single-pulse detector                               */
+/* ------------------------------------------------------------------ */
+
+/* Per-channel robust clip of uint8 samples [t0, t0+n) of each channel:
+ * med = lower median, mad = lower median of |x-med|, samples with
+ * |x-med| > (nsigma*1.4826f)*mad are replaced by med.  d_out may equal d_in.
+ * d_med/d_mad/d_nclipped: per-channel int32 outputs (may be NULL).
+ * Replaces the Pelican-LOFAR RFI_Clipper stage that precedes MdsmModule
+ * (src/pelican-mdsm/pipelines/src/MdsmPipeline.cpp:57-59) and the prototype
+ * src/prototypes/rfi_clipper.cu:8-67. */
+int mdsm_clip_u8(const uint8_t *d_in, int64_t in_stride, uint32_t nchans,
+                 int64_t t0, uint32_t n, float nsigma, uint8_t *d_out,
+                 int64_t out_stride, int32_t *d_med, int32_t *d_mad,
+                 int32_t *d_nclipped, void *stream);
+
+typedef struct {
+    uint32_t dm;     /* output row */
+    uint32_t t;      /* first sample of the boxcar */
+    uint32_t width;  /* boxcar width in samples */
+    float snr;
+} mdsm_cand;
+
+/* Boxcar single-pulse search of a DM x time block d_x [ndms][stride]
+ * (n valid samples per row).  Widths must be powers of two (<= 1024).
+ * Per row: mean and sigma over the n samples (FP64); boxcar sums as a
+ * pairwise FP64 tree; a candidate (d,t,w,snr) where
+ * snr = (S_w(t) - w*mean)/(sigma*sqrt(w)) > thr and S_w peaks at t
+ * (S_w(t) >= S_w(t-1), S_w(t) > S_w(t+1)).  The library zeroes *d_count,
+ * then appends candidates in arbitrary order to d_cands (capacity cap);
+ * *d_count receives the total found, which may exceed cap (the caller
+ * checks it after the stream completes).  d_mean/d_sigma: required [ndms]
+ * FP64 outputs, the row statistics.
+ * Replaces the detector thread's threshold search
+ * (src/mdsm/dedispersion_output.cpp:135-213) with the boxcar search of
+ * src/prototypes/singlepulse.cpp:41-146. */
+int mdsm_detect_boxcar(const float *d_x, uint32_t ndms, uint32_t n,
+                       int64_t stride, const uint32_t *widths, uint32_t nwidths,
+                       double thr, mdsm_cand *d_cands, uint32_t cap,
+                       uint32_t *d_count, double *d_mean, double *d_sigma,
+                       void *stream);
+
+/* Median-of-five threshold detector of the reference's output thread
+ * (src/mdsm/dedispersion_output.cpp:14-53,135-213): per row d, for
+ * l < n-4, m = median(x[l..l+4]); emit (d, l, 5, m/sigma) when
+ * m - mean >= thr*sigma and dm_values[d] > 1.0.  mean/sigma are the
+ * FP64 statistics over the whole block (all rows), narrowed to float as the
+ * reference's localmean/localrms.  d_stats: device scratch of 2+2*ndms
+ * doubles; d_stats[0..1] receive mean and sigma. */
+int mdsm_detect_median5(const float *d_x, uint32_t ndms, uint32_t n,
+                        int64_t stride, const float *d_dm_values, double thr,
+                        mdsm_cand *d_cands, uint32_t cap, uint32_t *d_count,
+                        double *d_stats, void *stream);
+
+/* ------------------------------------------------------------------ */
+/* Synthetic filterbanks (seeded, host side)                           */
+/* ------------------------------------------------------------------ */
+
+typedef struct {
+    double dm;        /* pc cm^-3 */
+    uint32_t t0;      /* arrival sample at channel 0 (top of band) */
+    uint32_t width;   /* samples */
+    float amp;        /* in units of the noise sigma, per channel */
+    uint32_t pad;
+} mdsm_pulse;
+
+typedef struct {
+    uint64_t seed;
+    int32_t dtype;      /* uint8: clip(round(mean + sigma*(z+pulse)),0,255) */
+    uint32_t nchans;
+    uint32_t nsamp;
+    float fch1, foff, tsamp;
+    float mean, sigma;  /* noise N(mean, sigma) before gain */
+    float gain_lo, gain_hi; /* per-channel gain U[lo,hi] (both 1: none) */
+    uint32_t npulses;
+    const mdsm_pulse *pulses;
+    uint32_t nthreads;  /* host threads (0 = hardware concurrency) */
+} mdsm_synth_spec;
+
+/* Fill host_out ([nchans][out_stride] of dtype) with seeded Gaussian noise
+ * plus pulses placed on the MDSM_SHIFT_NS track (nearest sample).  The value
+ * of every sample depends only on (seed, channel, sample). */
+int mdsm_synth_filterbank(const mdsm_synth_spec *spec, void *host_out,
+                          int64_t out_stride);
+
+/* ------------------------------------------------------------------ */
+/* Device plumbing                                                     */
+/* ------------------------------------------------------------------ */
+/* Memory, copies and streams, so that hosts need no second HIP runtime.
+ * They replace the reference worker's direct CUDA runtime calls:
+ * cudaSetDevice/cudaMalloc (src/mdsm/dedispersion_thread.cu:258-262), the
+ * H2D/D2H copies (:285,297,353-355), cudaMemset (:324) and the device
+ * enumeration of initialise_devices (:4-62). */
+
+typedef struct {
+    char name[256];
+    char arch[64];
+    uint64_t total_mem, free_mem;
+    int32_t cus, clock_khz, mem_clock_khz, mem_bus_bits;
+    uint32_t lds_per_block;
+    uint32_t pad;
+} mdsm_device_props;
+
+enum { MDSM_H2D = 1, MDSM_D2H = 2, MDSM_D2D = 3 };
+
+int mdsm_device_count(int *count);
+int mdsm_device_set(int device);
+int mdsm_device_query(int device, mdsm_device_props *props);
+int mdsm_device_malloc(void **ptr, uint64_t bytes);
+int mdsm_device_free(void *ptr);
+int mdsm_host_malloc_pinned(void **ptr, uint64_t bytes);
+int mdsm_host_free_pinned(void *ptr);
+/* stream == NULL: synchronous copy. */
+int mdsm_memcpy(void *dst, const void *src, uint64_t bytes, int kind, void *stream);
+int mdsm_memcpy_2d(void *dst, uint64_t dpitch, const void *src, uint64_t spitch,
+                   uint64_t width, uint64_t rows, int kind, void *stream);
+/* Fill count uint32 words with value by a kernel (graph-capturable). */
+int mdsm_fill_u32(void *dst, uint32_t value, uint64_t count, void *stream);
+int mdsm_stream_create(void **stream);
+int mdsm_stream_destroy(void *stream);
+int mdsm_stream_synchronize(void *stream);
+int mdsm_device_synchronize(void);
+
+/* ------------------------------------------------------------------ */
+/* Kernel timing and errors                                            */
+/* ------------------------------------------------------------------ */
+
+/* When enabled, every kernel the library launches is bracketed by a pair
+ * of hipEvents on its launch stream.  mdsm_timing_get synchronises those
+ * events and returns the summed duration (ms) and launch count of kernels
+ * named `kernel` ("dedisperse", "decimate", "clip", "detect", ...). */
+int mdsm_timing_enable(int on);
+int mdsm_timing_reset(void);
+int mdsm_timing_get(const char *kernel, double *total_ms, uint64_t *launches);
+
+const char *mdsm_last_error(void);
+const char *mdsm_version(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MDSM_C_H */

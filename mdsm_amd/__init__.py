@@ -1,0 +1,445 @@
+"""MI355X-native incoherent-dedispersion path of MDSM (host bindings).
+
+Python bindings over the C ABI of ``libmdsm_hip.so`` (include/mdsm_c.h).  The
+compute path is the gfx950 HIP library; nothing here computes samples on the
+CPU.  If the library has not been built, every entry point raises: there is
+no fallback.
+
+    from mdsm_amd import Plan, DeviceBuffer, synth_filterbank
+    plan = Plan.brute(nchans=256, fch1=1500.0, foff=-0.390625, tsamp=64e-6,
+                      nsamp_in=65536, dtype="f32",
+                      ranges=[(0.0, 0.2, 512, 1)])
+    outs = plan.dedisperse_host(x)      # list of [ndms_r][nout_r] arrays
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+from dataclasses import dataclass
+from typing import Sequence
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libmdsm_hip.so")
+
+U8, F16, F32 = 0, 1, 2
+SHIFT_REF, SHIFT_NS = 0, 1
+H2D, D2H, D2D = 1, 2, 3
+DTYPES = {"u8": U8, "f16": F16, "f32": F32}
+NP_DTYPE = {U8: np.uint8, F16: np.float16, F32: np.float32}
+STATUS = {0: "OK", -1: "INVALID", -2: "NOMEM", -3: "DEVICE", -4: "CAPACITY", -5: "BUDGET",
+          -6: "STATE", -7: "PARSE", -8: "AGAIN"}
+
+
+class MdsmError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"mdsm error {code} ({STATUS.get(code, '?')}): {msg}")
+        self.code = code
+
+
+# ---------------------------------------------------------------- C structs
+class Obs(C.Structure):
+    _fields_ = [("nchans", C.c_uint32), ("fch1", C.c_float), ("foff", C.c_float),
+                ("tsamp", C.c_float), ("nsamp_in", C.c_uint32), ("dtype", C.c_int32),
+                ("shift_rule", C.c_int32)]
+
+
+class Range(C.Structure):
+    _fields_ = [("lowdm", C.c_float), ("dmstep", C.c_float), ("ndms", C.c_uint32),
+                ("downsample", C.c_uint32)]
+
+
+class Pass(C.Structure):
+    _fields_ = [("lowdm", C.c_float), ("highdm", C.c_float), ("dmstep", C.c_float),
+                ("sub_dmstep", C.c_float), ("binsize", C.c_uint32), ("ndms", C.c_uint32),
+                ("ncalls", C.c_uint32), ("calldms", C.c_uint32)]
+
+
+class PlanInfo(C.Structure):
+    _fields_ = [("nranges", C.c_uint32), ("ndms_total", C.c_uint32), ("maxshift", C.c_uint32),
+                ("nchans", C.c_uint32), ("out_elems", C.c_uint64), ("work_bytes", C.c_uint64),
+                ("dm_chan_samples", C.c_double), ("adds", C.c_double), ("bytes", C.c_double),
+                ("is_subband", C.c_uint32), ("nblocks", C.c_uint32)]
+
+
+class RangeInfo(C.Structure):
+    _fields_ = [("lowdm", C.c_float), ("dmstep", C.c_float), ("ndms", C.c_uint32),
+                ("downsample", C.c_uint32), ("nout", C.c_uint32), ("maxshift", C.c_uint32),
+                ("out_offset", C.c_uint64), ("first_dm", C.c_uint32), ("ncalls", C.c_uint32)]
+
+
+class Cand(C.Structure):
+    _fields_ = [("dm", C.c_uint32), ("t", C.c_uint32), ("width", C.c_uint32),
+                ("snr", C.c_float)]
+
+
+CAND_DTYPE = np.dtype([("dm", np.uint32), ("t", np.uint32), ("width", np.uint32),
+                       ("snr", np.float32)])
+
+
+class Pulse(C.Structure):
+    _fields_ = [("dm", C.c_double), ("t0", C.c_uint32), ("width", C.c_uint32),
+                ("amp", C.c_float), ("pad", C.c_uint32)]
+
+
+class SynthSpec(C.Structure):
+    _fields_ = [("seed", C.c_uint64), ("dtype", C.c_int32), ("nchans", C.c_uint32),
+                ("nsamp", C.c_uint32), ("fch1", C.c_float), ("foff", C.c_float),
+                ("tsamp", C.c_float), ("mean", C.c_float), ("sigma", C.c_float),
+                ("gain_lo", C.c_float), ("gain_hi", C.c_float), ("npulses", C.c_uint32),
+                ("pulses", C.POINTER(Pulse)), ("nthreads", C.c_uint32)]
+
+
+class DeviceProps(C.Structure):
+    _fields_ = [("name", C.c_char * 256), ("arch", C.c_char * 64), ("total_mem", C.c_uint64),
+                ("free_mem", C.c_uint64), ("cus", C.c_int32), ("clock_khz", C.c_int32),
+                ("mem_clock_khz", C.c_int32), ("mem_bus_bits", C.c_int32),
+                ("lds_per_block", C.c_uint32), ("pad", C.c_uint32)]
+
+
+# ---------------------------------------------------------------- loading
+_lib = None
+
+
+def lib():
+    """The loaded libmdsm_hip.so; raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with python -m mdsm_amd.build "
+                           "(there is no CPU fallback)")
+    L = C.CDLL(LIB_PATH)
+    vp, i32, u32, u64, i64, f32, f64 = (C.c_void_p, C.c_int32, C.c_uint32, C.c_uint64,
+                                        C.c_int64, C.c_float, C.c_double)
+    sig = {
+        "mdsm_plan_create": [C.POINTER(Obs), C.POINTER(Range), u32, C.POINTER(vp)],
+        "mdsm_plan_create_subband": [C.POINTER(Obs), u32, C.POINTER(Pass), u32, C.POINTER(vp)],
+        "mdsm_plan_get_info": [vp, C.POINTER(PlanInfo)],
+        "mdsm_plan_get_range": [vp, u32, C.POINTER(RangeInfo)],
+        "mdsm_plan_shift_table": [vp, u32, vp],
+        "mdsm_plan_fine_table": [vp, u32, vp],
+        "mdsm_plan_dm_values": [vp, vp],
+        "mdsm_dedisperse": [vp, vp, i64, vp, vp, vp],
+        "mdsm_decimate": [vp, i32, u32, i64, u32, u32, vp, i64, vp],
+        "mdsm_clip_u8": [vp, i64, u32, i64, u32, f32, vp, i64, vp, vp, vp, vp],
+        "mdsm_detect_boxcar": [vp, u32, u32, i64, vp, u32, f64, vp, u32, vp, vp, vp, vp],
+        "mdsm_detect_median5": [vp, u32, u32, i64, vp, f64, vp, u32, vp, vp, vp],
+        "mdsm_synth_filterbank": [C.POINTER(SynthSpec), vp, i64],
+        "mdsm_timing_enable": [C.c_int],
+        "mdsm_timing_reset": [],
+        "mdsm_timing_get": [C.c_char_p, C.POINTER(f64), C.POINTER(u64)],
+        "mdsm_device_count": [C.POINTER(C.c_int)],
+        "mdsm_device_set": [C.c_int],
+        "mdsm_device_query": [C.c_int, C.POINTER(DeviceProps)],
+        "mdsm_device_malloc": [C.POINTER(vp), u64],
+        "mdsm_device_free": [vp],
+        "mdsm_host_malloc_pinned": [C.POINTER(vp), u64],
+        "mdsm_host_free_pinned": [vp],
+        "mdsm_memcpy": [vp, vp, u64, C.c_int, vp],
+        "mdsm_memcpy_2d": [vp, u64, vp, u64, u64, u64, C.c_int, vp],
+        "mdsm_fill_u32": [vp, u32, u64, vp],
+        "mdsm_stream_create": [C.POINTER(vp)],
+        "mdsm_stream_destroy": [vp],
+        "mdsm_stream_synchronize": [vp],
+        "mdsm_device_synchronize": [],
+    }
+    for name, args in sig.items():
+        fn = getattr(L, name)
+        fn.argtypes = args
+        fn.restype = C.c_int
+    L.mdsm_plan_destroy.argtypes = [vp]
+    L.mdsm_plan_destroy.restype = None
+    L.mdsm_last_error.restype = C.c_char_p
+    L.mdsm_version.restype = C.c_char_p
+    _lib = L
+    return L
+
+
+def check(rc):
+    if rc != 0:
+        raise MdsmError(rc, lib().mdsm_last_error().decode(errors="replace"))
+    return rc
+
+
+def version() -> str:
+    return lib().mdsm_version().decode()
+
+
+# ---------------------------------------------------------------- device plumbing
+def device_count() -> int:
+    n = C.c_int(0)
+    rc = lib().mdsm_device_count(C.byref(n))
+    return n.value if rc == 0 else 0
+
+
+def device_props(dev: int = 0) -> dict:
+    p = DeviceProps()
+    check(lib().mdsm_device_query(dev, C.byref(p)))
+    return {"name": p.name.decode(), "arch": p.arch.decode(), "total_mem": p.total_mem,
+            "free_mem": p.free_mem, "cus": p.cus, "clock_khz": p.clock_khz,
+            "mem_clock_khz": p.mem_clock_khz, "mem_bus_bits": p.mem_bus_bits,
+            "lds_per_block": p.lds_per_block}
+
+
+def synchronize():
+    check(lib().mdsm_device_synchronize())
+
+
+class Stream:
+    def __init__(self):
+        h = C.c_void_p()
+        check(lib().mdsm_stream_create(C.byref(h)))
+        self.handle = h.value
+
+    def synchronize(self):
+        check(lib().mdsm_stream_synchronize(self.handle))
+
+    def close(self):
+        if self.handle:
+            lib().mdsm_stream_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def _sh(stream):
+    if stream is None:
+        return None
+    return stream.handle if isinstance(stream, Stream) else stream
+
+
+class DeviceBuffer:
+    """A device allocation owned by this object (freed on close/__del__)."""
+
+    def __init__(self, nbytes: int):
+        h = C.c_void_p()
+        check(lib().mdsm_device_malloc(C.byref(h), int(nbytes)))
+        self.ptr = h.value
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_host(cls, arr: np.ndarray, stream=None) -> "DeviceBuffer":
+        arr = np.ascontiguousarray(arr)
+        b = cls(arr.nbytes)
+        b.upload(arr, stream)
+        return b
+
+    def upload(self, arr: np.ndarray, stream=None, offset: int = 0):
+        arr = np.ascontiguousarray(arr)
+        if offset + arr.nbytes > self.nbytes:
+            raise ValueError("upload exceeds buffer")
+        check(lib().mdsm_memcpy(self.ptr + offset, arr.ctypes.data, arr.nbytes, H2D, _sh(stream)))
+
+    def download(self, dtype, shape, offset: int = 0) -> np.ndarray:
+        out = np.empty(shape, dtype)
+        if offset + out.nbytes > self.nbytes:
+            raise ValueError("download exceeds buffer")
+        check(lib().mdsm_memcpy(out.ctypes.data, self.ptr + offset, out.nbytes, D2H, None))
+        return out
+
+    def fill_u32(self, value: int = 0, count: int | None = None, stream=None):
+        n = self.nbytes // 4 if count is None else count
+        check(lib().mdsm_fill_u32(self.ptr, value, n, _sh(stream)))
+
+    def close(self):
+        if self.ptr:
+            lib().mdsm_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- timing
+def timing_enable(on: bool = True):
+    check(lib().mdsm_timing_enable(1 if on else 0))
+
+
+def timing_reset():
+    check(lib().mdsm_timing_reset())
+
+
+def timing_get(kernel: str):
+    ms, n = C.c_double(), C.c_uint64()
+    check(lib().mdsm_timing_get(kernel.encode(), C.byref(ms), C.byref(n)))
+    return ms.value, n.value
+
+
+# ---------------------------------------------------------------- plans
+@dataclass
+class RangeLayout:
+    lowdm: float
+    dmstep: float
+    ndms: int
+    downsample: int
+    nout: int
+    maxshift: int
+    out_offset: int
+    first_dm: int
+    ncalls: int
+
+
+class Plan:
+    """A dedispersion plan: brute-force multi-range or two-stage subband."""
+
+    def __init__(self, handle, obs: Obs):
+        self._h = handle
+        self.obs = obs
+        info = PlanInfo()
+        check(lib().mdsm_plan_get_info(self._h, C.byref(info)))
+        self.info = info
+        self.ranges: list[RangeLayout] = []
+        for r in range(info.nranges):
+            ri = RangeInfo()
+            check(lib().mdsm_plan_get_range(self._h, r, C.byref(ri)))
+            self.ranges.append(RangeLayout(ri.lowdm, ri.dmstep, ri.ndms, ri.downsample, ri.nout,
+                                           ri.maxshift, ri.out_offset, ri.first_dm, ri.ncalls))
+        self._work = None
+
+    @staticmethod
+    def make_obs(nchans, fch1, foff, tsamp, nsamp_in, dtype="f32", shift_rule=SHIFT_REF) -> Obs:
+        dt = DTYPES[dtype] if isinstance(dtype, str) else int(dtype)
+        return Obs(nchans, fch1, foff, tsamp, nsamp_in, dt, shift_rule)
+
+    @classmethod
+    def brute(cls, nchans, fch1, foff, tsamp, nsamp_in, dtype="f32",
+              ranges: Sequence[tuple] = ((0.0, 1.0, 1, 1),), shift_rule=SHIFT_REF) -> "Plan":
+        obs = cls.make_obs(nchans, fch1, foff, tsamp, nsamp_in, dtype, shift_rule)
+        arr = (Range * len(ranges))(*[Range(float(a), float(b), int(n), int(k))
+                                      for (a, b, n, k) in ranges])
+        h = C.c_void_p()
+        check(lib().mdsm_plan_create(C.byref(obs), arr, len(ranges), C.byref(h)))
+        return cls(h.value, obs)
+
+    @classmethod
+    def subband(cls, nchans, fch1, foff, tsamp, nsamp_in, nsubs, passes: Sequence[dict],
+                dtype="f32", shift_rule=SHIFT_REF) -> "Plan":
+        obs = cls.make_obs(nchans, fch1, foff, tsamp, nsamp_in, dtype, shift_rule)
+        arr = (Pass * len(passes))()
+        for i, p in enumerate(passes):
+            arr[i] = Pass(p.get("lowdm", 0.0), p.get("highdm", 0.0), p["dmstep"],
+                          p["sub_dmstep"], p.get("binsize", 1), p["ncalls"] * p["calldms"],
+                          p["ncalls"], p["calldms"])
+        h = C.c_void_p()
+        check(lib().mdsm_plan_create_subband(C.byref(obs), nsubs, arr, len(passes), C.byref(h)))
+        return cls(h.value, obs)
+
+    @property
+    def handle(self):
+        return self._h
+
+    def shift_table(self, r: int) -> np.ndarray:
+        R = self.ranges[r]
+        out = np.empty((R.ncalls, self.obs.nchans), np.int32)
+        check(lib().mdsm_plan_shift_table(self._h, r, out.ctypes.data))
+        return out
+
+    def fine_table(self, r: int, nsubs: int) -> np.ndarray:
+        R = self.ranges[r]
+        out = np.empty((R.ndms, nsubs), np.int32)
+        check(lib().mdsm_plan_fine_table(self._h, r, out.ctypes.data))
+        return out
+
+    def dm_values(self) -> np.ndarray:
+        out = np.empty(self.info.ndms_total, np.float32)
+        check(lib().mdsm_plan_dm_values(self._h, out.ctypes.data))
+        return out
+
+    def dedisperse(self, d_in, in_stride: int, d_out, d_work=None, stream=None):
+        pin = d_in.ptr if isinstance(d_in, DeviceBuffer) else d_in
+        pout = d_out.ptr if isinstance(d_out, DeviceBuffer) else d_out
+        pw = d_work.ptr if isinstance(d_work, DeviceBuffer) else d_work
+        check(lib().mdsm_dedisperse(self._h, pin, int(in_stride), pout, pw, _sh(stream)))
+
+    def split_output(self, flat: np.ndarray) -> list:
+        return [flat[R.out_offset:R.out_offset + R.ndms * R.nout].reshape(R.ndms, R.nout)
+                for R in self.ranges]
+
+    def dedisperse_host(self, x: np.ndarray) -> list:
+        """Upload a channel-major block, dedisperse on the GPU, download."""
+        x = np.ascontiguousarray(x, NP_DTYPE[self.obs.dtype])
+        assert x.shape[0] == self.obs.nchans and x.shape[1] >= self.obs.nsamp_in
+        din = DeviceBuffer.from_host(x)
+        dout = DeviceBuffer(self.info.out_elems * 4)
+        try:
+            self.dedisperse(din, x.shape[1], dout)
+            synchronize()
+            flat = dout.download(np.float32, (self.info.out_elems,))
+        finally:
+            din.close()
+            dout.close()
+        return self.split_output(flat)
+
+    def close(self):
+        if self._h:
+            lib().mdsm_plan_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+# ---------------------------------------------------------------- clipper / detectors
+def clip_u8(d_in, in_stride, nchans, t0, n, nsigma, d_out, out_stride, d_med=None, d_mad=None,
+            d_ncl=None, stream=None):
+    g = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
+    check(lib().mdsm_clip_u8(g(d_in), in_stride, nchans, t0, n, nsigma, g(d_out), out_stride,
+                             g(d_med), g(d_mad), g(d_ncl), _sh(stream)))
+
+
+def detect_boxcar(d_x, ndms, n, stride, widths, thr, d_cands, cap, d_count, d_mean, d_sigma,
+                  stream=None):
+    g = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
+    w = (C.c_uint32 * len(widths))(*widths)
+    check(lib().mdsm_detect_boxcar(g(d_x), ndms, n, stride, w, len(widths), thr, g(d_cands), cap,
+                                   g(d_count), g(d_mean), g(d_sigma), _sh(stream)))
+
+
+def detect_median5(d_x, ndms, n, stride, d_dmv, thr, d_cands, cap, d_count, d_stats,
+                   stream=None):
+    g = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
+    check(lib().mdsm_detect_median5(g(d_x), ndms, n, stride, g(d_dmv), thr, g(d_cands), cap,
+                                    g(d_count), g(d_stats), _sh(stream)))
+
+
+def read_candidates(d_cands, d_count, cap) -> np.ndarray:
+    """Download and sort (dm, width, t) a candidate list written by a detector."""
+    cnt = int(d_count.download(np.uint32, (1,))[0])
+    m = min(cnt, cap)
+    arr = d_cands.download(CAND_DTYPE, (m,)) if m else np.zeros(0, CAND_DTYPE)
+    arr = np.sort(arr, order=["dm", "width", "t"])
+    if cnt > cap:
+        raise MdsmError(-4, f"candidate overflow: {cnt} found, capacity {cap}")
+    return arr
+
+
+# ---------------------------------------------------------------- synthetic data
+def synth_filterbank(nchans, nsamp, dtype="u8", seed=1, fch1=1500.0, foff=-0.390625,
+                     tsamp=64e-6, mean=0.0, sigma=1.0, gain=(1.0, 1.0), pulses=(),
+                     stride=None, nthreads=0) -> np.ndarray:
+    """Seeded channel-major filterbank [nchans][stride] (host).
+
+    pulses: sequence of (dm, t0, width, amp); amp in units of sigma.  Pulses
+    follow the nearest-sample FP64 delay track (shift rule NS)."""
+    dt = DTYPES[dtype] if isinstance(dtype, str) else int(dtype)
+    stride = nsamp if stride is None else stride
+    out = np.empty((nchans, stride), NP_DTYPE[dt])
+    P = (Pulse * max(1, len(pulses)))()
+    for i, (dm, t0, w, a) in enumerate(pulses):
+        P[i] = Pulse(float(dm), int(t0), int(w), float(a), 0)
+    spec = SynthSpec(int(seed), dt, nchans, nsamp, fch1, foff, tsamp, mean, sigma, gain[0],
+                     gain[1], len(pulses), P, nthreads)
+    check(lib().mdsm_synth_filterbank(C.byref(spec), out.ctypes.data, stride))
+    return out

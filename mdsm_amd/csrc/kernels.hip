@@ -1,0 +1,570 @@
+// gfx950 kernels of the dedispersion path.
+//
+// dedisperse_kernel: one workgroup (4 waves) per work item = (block of
+//   kDB=32 DM trials sharing one set of input rows, tile of 256 output
+//   samples).  Channels are staged 16 at a time into LDS as float32 rows
+//   covering [t0 + abase_c, t0 + abase_c + W); each wave holds 8 trials x 4
+//   samples of accumulators in registers and reads its per-(row, trial)
+//   offsets as wave-uniform scalars.  Work items are ordered tile-major and
+//   remapped so that each XCD (8 per MI355X, round-robin dispatch) receives a
+//   contiguous run: blocks sharing an input tile meet in the same L2.
+#include <hip/hip_runtime.h>
+
+#include "mdsm_internal.h"
+
+namespace mdsm {
+
+struct DedispArgs {
+    const BlockDesc *blocks;
+    const SegDesc *segs;
+    const int32_t *rel;
+    const int32_t *abase;
+    const void *in;
+    float *out;
+    char *work;
+    int64_t in_stride;
+    int64_t items;
+    int32_t seg_begin;
+    int32_t nseg;
+};
+
+template <int DT>
+__device__ __forceinline__ float load_sample(const void *base, int64_t i)
+{
+    if constexpr (DT == DT_U8) {
+        return (float)((const uint8_t *)base)[i];
+    } else if constexpr (DT == DT_U16) {
+        return (float)((const uint16_t *)base)[i];
+    } else if constexpr (DT == DT_F16) {
+        return (float)((const _Float16 *)base)[i];
+    } else {
+        return ((const float *)base)[i];
+    }
+}
+
+// Stage rows [c0, c0+cc) of the block into LDS: wave w fills rows w, w+4, ...
+// Row i holds columns abase[c0+i] + t0 + [0, W) (zero beyond in_len).
+template <int DT>
+__device__ __forceinline__ void load_tile(float *tile, const void *src, int64_t in_off,
+                                          int64_t row0, int64_t rs, const int32_t *ab, int t0,
+                                          int W, int cc, int in_len, int wave, int lane)
+{
+    for (int i = wave; i < cc; i += kWaves) {
+        const int64_t rowbase = in_off + (row0 + i) * rs;
+        const int col0 = ab[i] + t0;
+        float *dst = tile + i * W;
+        for (int e = lane; e < W; e += 64) {
+            const int col = col0 + e;
+            dst[e] = col < in_len ? load_sample<DT>(src, rowbase + col) : 0.0f;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) dedisperse_kernel(DedispArgs a)
+{
+    extern __shared__ float tile[];
+
+    // XCD-aware remap: XCD x = blockIdx % 8 gets a contiguous run of items.
+    const int64_t g = blockIdx.x;
+    const int64_t q = a.items >> 3, rem = a.items & 7;
+    const int64_t xcd = g & 7, loc = g >> 3;
+    const int64_t f = xcd * q + (xcd < rem ? xcd : rem) + loc;
+
+    // Segment lookup (wave-uniform binary search over item_start).
+    int lo = a.seg_begin, hi = a.seg_begin + a.nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.segs[mid].item_start <= f)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const SegDesc sg = a.segs[lo];
+    const int64_t fl = f - sg.item_start;
+    const int tile_i = (int)(fl / sg.nblocks);
+    const int bi = sg.first_block + (int)(fl - (int64_t)tile_i * sg.nblocks);
+    const BlockDesc bd = a.blocks[bi];
+
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int t0 = tile_i * kTile;
+    const int W = bd.W;
+    const int64_t rs = bd.src == BUF_IN ? a.in_stride : bd.row_stride;
+    const void *src = bd.src == BUF_IN ? a.in : (const void *)a.work;
+    const int32_t *relw = a.rel + bd.tab_off + wave * kRPW;
+    const int32_t *ab = a.abase + bd.base_off;
+
+    float acc[kRPW][kPPL];
+#pragma unroll
+    for (int r = 0; r < kRPW; ++r)
+#pragma unroll
+        for (int p = 0; p < kPPL; ++p) acc[r][p] = 0.0f;
+
+    for (int c0 = 0; c0 < bd.nrows; c0 += kCC) {
+        const int cc = min(kCC, bd.nrows - c0);
+        __syncthreads();
+        switch (bd.dtype) {
+        case DT_U8:
+            load_tile<DT_U8>(tile, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, W, cc, bd.in_len,
+                             wave, lane);
+            break;
+        case DT_U16:
+            load_tile<DT_U16>(tile, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, W, cc,
+                              bd.in_len, wave, lane);
+            break;
+        case DT_F16:
+            load_tile<DT_F16>(tile, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, W, cc,
+                              bd.in_len, wave, lane);
+            break;
+        default:
+            load_tile<DT_F32>(tile, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, W, cc,
+                              bd.in_len, wave, lane);
+            break;
+        }
+        __syncthreads();
+        for (int i = 0; i < cc; ++i) {
+            const float *row = tile + i * W + lane;
+            const int32_t *rl = relw + (c0 + i) * kDB;
+#pragma unroll
+            for (int r = 0; r < kRPW; ++r) {
+                const float *qp = row + rl[r];
+#pragma unroll
+                for (int p = 0; p < kPPL; ++p) acc[r][p] += qp[64 * p];
+            }
+        }
+    }
+
+    float *ob = (bd.dst == BUF_OUT ? a.out : (float *)a.work) + bd.out_off;
+    const float kf = (float)bd.k;
+#pragma unroll
+    for (int r = 0; r < kRPW; ++r) {
+        const int j = wave * kRPW + r;
+        if (j < bd.ntrials) {
+            float *o = ob + (int64_t)j * bd.out_trial_stride;
+#pragma unroll
+            for (int p = 0; p < kPPL; ++p) {
+                const int t = t0 + lane + 64 * p;
+                if (t < bd.nout) o[t] = bd.k > 1 ? __fdiv_rn(acc[r][p], kf) : acc[r][p];
+            }
+        }
+    }
+}
+
+hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in,
+                             int64_t in_stride, float *out, void *work, hipStream_t s)
+{
+    if (st.items <= 0) return hipSuccess;
+    if (st.items > 0x7fffffffLL) return hipErrorInvalidValue;
+    DedispArgs a;
+    a.blocks = p->dev.blocks;
+    a.segs = p->dev.segs;
+    a.rel = p->dev.rel;
+    a.abase = p->dev.abase;
+    a.in = in;
+    a.out = out;
+    a.work = (char *)work;
+    a.in_stride = in_stride;
+    a.items = st.items;
+    a.seg_begin = st.seg_begin;
+    a.nseg = st.seg_end - st.seg_begin;
+    const size_t lds = (size_t)kCC * st.wmax * sizeof(float);
+    hipLaunchKernelGGL(dedisperse_kernel, dim3((unsigned)st.items), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Decimation: out[c][t] = sum_{j<k} in[c][k t + j]
+// ---------------------------------------------------------------------------
+
+template <int DTI, int DTO>
+__global__ void __launch_bounds__(256)
+    decimate_kernel(const void *in, int64_t in_stride, uint32_t nsamp_out, uint32_t k, void *out,
+                    int64_t out_stride)
+{
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    if (t >= nsamp_out) return;
+    const int64_t base = c * in_stride + (int64_t)t * k;
+    if constexpr (DTI == DT_U8) {
+        uint32_t acc = 0;
+        for (uint32_t j = 0; j < k; ++j) acc += ((const uint8_t *)in)[base + j];
+        ((uint16_t *)out)[c * out_stride + t] = (uint16_t)acc;
+    } else {
+        float acc = 0.0f;
+        for (uint32_t j = 0; j < k; ++j) acc += load_sample<DTI>(in, base + j);
+        ((float *)out)[c * out_stride + t] = acc;
+    }
+}
+
+hipError_t launch_decimate(const void *in, int dtype_in, uint32_t nchans, int64_t in_stride,
+                           uint32_t nsamp, uint32_t k, void *out, int dtype_out,
+                           int64_t out_stride, hipStream_t s)
+{
+    const uint32_t nout = nsamp / k;
+    if (nout == 0 || nchans == 0) return hipSuccess;
+    dim3 grid((nout + 255) / 256, nchans);
+    if (dtype_in == DT_U8 && dtype_out == DT_U16)
+        hipLaunchKernelGGL((decimate_kernel<DT_U8, DT_U16>), grid, dim3(256), 0, s, in, in_stride,
+                           nout, k, out, out_stride);
+    else if (dtype_in == DT_F16 && dtype_out == DT_F32)
+        hipLaunchKernelGGL((decimate_kernel<DT_F16, DT_F32>), grid, dim3(256), 0, s, in,
+                           in_stride, nout, k, out, out_stride);
+    else if (dtype_in == DT_F32 && dtype_out == DT_F32)
+        hipLaunchKernelGGL((decimate_kernel<DT_F32, DT_F32>), grid, dim3(256), 0, s, in,
+                           in_stride, nout, k, out, out_stride);
+    else
+        return hipErrorInvalidValue;
+    return hipGetLastError();
+}
+
+// Small fill kernel: used instead of hipMemsetAsync so that the library's
+// launch sequences can be captured into HIP graphs.
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t n, uint32_t v)
+{
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((n + 255) / 256 > 1024 ? 1024 : (n + 255) / 256),
+                       dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Wave-level helpers (64 lanes)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t wave_incl_scan(uint32_t v, int lane)
+{
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const uint32_t o = __shfl_up(v, d, 64);
+        if (lane >= d) v += o;
+    }
+    return v;
+}
+
+__device__ __forceinline__ double wave_sum_f64(double v)
+{
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) v += __shfl_xor(v, d, 64);
+    return v;
+}
+
+// Rank query on a 256-bin LDS histogram: the smallest value v whose
+// cumulative count exceeds `rank`.  Executed by wave 0 (4 bins per lane).
+__device__ int hist_rank(const uint32_t *hist, uint32_t rank, int lane)
+{
+    const uint32_t h0 = hist[4 * lane], h1 = hist[4 * lane + 1], h2 = hist[4 * lane + 2],
+                   h3 = hist[4 * lane + 3];
+    const uint32_t mine = h0 + h1 + h2 + h3;
+    const uint32_t incl = wave_incl_scan(mine, lane);
+    const uint32_t excl = incl - mine;
+    // lane holding the answer: excl <= rank < incl
+    const bool here = excl <= rank && rank < incl;
+    int v = 255;
+    if (here) {
+        uint32_t acc = excl + h0;
+        v = 4 * lane;
+        if (acc <= rank) {
+            acc += h1;
+            v++;
+            if (acc <= rank) {
+                acc += h2;
+                v++;
+                if (acc <= rank) v++;
+            }
+        }
+    }
+    // broadcast: the unique lane with `here`, found via ballot
+    const uint64_t m = __ballot(here);
+    const int src = m ? (int)__ffsll((unsigned long long)m) - 1 : 0;
+    return __shfl(v, src, 64);
+}
+
+// ---------------------------------------------------------------------------
+// RFI clipper (uint8): one workgroup per channel.
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256)
+    clip_u8_kernel(const uint8_t *in, int64_t in_stride, int64_t t0, uint32_t n, float nsigma,
+                   uint8_t *out, int64_t out_stride, int32_t *med_out, int32_t *mad_out,
+                   int32_t *ncl_out)
+{
+    __shared__ uint32_t hist[256];
+    __shared__ int sh_med, sh_mad;
+    __shared__ uint32_t sh_cnt;
+    const int tid = threadIdx.x, lane = tid & 63;
+    const int64_t c = blockIdx.x;
+    const uint8_t *x = in + c * in_stride + t0;
+    uint8_t *o = out + c * out_stride + t0;
+    const uint32_t rank = (n - 1) / 2;
+
+    hist[tid] = 0;
+    if (tid == 0) sh_cnt = 0;
+    __syncthreads();
+    for (uint32_t t = tid; t < n; t += 256) atomicAdd(&hist[x[t]], 1u);
+    __syncthreads();
+    if (tid < 64) {
+        const int m = hist_rank(hist, rank, lane);
+        if (lane == 0) sh_med = m;
+    }
+    __syncthreads();
+    const int med = sh_med;
+    hist[tid] = 0;
+    __syncthreads();
+    for (uint32_t t = tid; t < n; t += 256) atomicAdd(&hist[abs((int)x[t] - med)], 1u);
+    __syncthreads();
+    if (tid < 64) {
+        const int m = hist_rank(hist, rank, lane);
+        if (lane == 0) sh_mad = m;
+    }
+    __syncthreads();
+    const int mad = sh_mad;
+    const float thr = (nsigma * 1.4826f) * (float)mad;
+    uint32_t mine = 0;
+    for (uint32_t t = tid; t < n; t += 256) {
+        const int v = x[t];
+        const bool clip = (float)abs(v - med) > thr;
+        mine += clip;
+        o[t] = clip ? (uint8_t)med : (uint8_t)v;
+    }
+    // wave reduction of the clip count, then one LDS atomic per wave
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) mine += __shfl_xor(mine, d, 64);
+    if (lane == 0) atomicAdd(&sh_cnt, mine);
+    __syncthreads();
+    if (tid == 0) {
+        if (med_out) med_out[c] = med;
+        if (mad_out) mad_out[c] = mad;
+        if (ncl_out) ncl_out[c] = (int32_t)sh_cnt;
+    }
+}
+
+hipError_t launch_clip_u8(const uint8_t *in, int64_t in_stride, uint32_t nchans, int64_t t0,
+                          uint32_t n, float nsigma, uint8_t *out, int64_t out_stride,
+                          int32_t *med, int32_t *mad, int32_t *ncl, hipStream_t s)
+{
+    if (nchans == 0 || n == 0) return hipSuccess;
+    hipLaunchKernelGGL(clip_u8_kernel, dim3(nchans), dim3(256), 0, s, in, in_stride, t0, n,
+                       nsigma, out, out_stride, med, mad, ncl);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Row statistics (FP64): one workgroup per row.  Writes sums s1, s2.
+// ---------------------------------------------------------------------------
+
+__global__ void __launch_bounds__(256)
+    row_sums_kernel(const float *x, uint32_t n, int64_t stride, double *s1_out, double *s2_out)
+{
+    __shared__ double p1[4], p2[4];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const float *row = x + (int64_t)blockIdx.x * stride;
+    double s1 = 0.0, s2 = 0.0;
+    for (uint32_t t = tid; t < n; t += 256) {
+        const double v = (double)row[t];
+        s1 += v;
+        s2 += v * v;
+    }
+    s1 = wave_sum_f64(s1);
+    s2 = wave_sum_f64(s2);
+    if (lane == 0) {
+        p1[wave] = s1;
+        p2[wave] = s2;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        s1_out[blockIdx.x] = (p1[0] + p1[1]) + (p1[2] + p1[3]);
+        s2_out[blockIdx.x] = (p2[0] + p2[1]) + (p2[2] + p2[3]);
+    }
+}
+
+// mean = s1/n, sigma = sqrt(s2/n - mean*mean) (each a single IEEE operation;
+// this file is compiled with -ffp-contract=off).
+__global__ void row_stats_finish(uint32_t ndms, uint32_t n, double *mean, double *sigma)
+{
+    const uint32_t d = blockIdx.x * 256u + threadIdx.x;
+    if (d >= ndms) return;
+    const double m = mean[d] / (double)n;
+    const double ex2 = sigma[d] / (double)n;
+    const double mm = m * m;
+    double var = ex2 - mm;
+    if (var < 0.0) var = 0.0;
+    mean[d] = m;
+    sigma[d] = sqrt(var);
+}
+
+// ---------------------------------------------------------------------------
+// Boxcar detector: grid (tiles of kBT samples, rows).
+// ---------------------------------------------------------------------------
+
+constexpr int kBT = 1024;           // samples per tile
+constexpr int kMaxBoxW = 1024;      // widest boxcar
+constexpr int kBoxL = kBT + kMaxBoxW + 1;
+
+__global__ void __launch_bounds__(256)
+    boxcar_kernel(const float *x, uint32_t n, int64_t stride, uint32_t wmask, uint32_t maxw,
+                  double thr, SqrtW sqrtw, mdsm_cand *cands, uint32_t cap,
+                  uint32_t *count, const double *mean, const double *sigma)
+{
+    __shared__ double buf[2][kBoxL];
+    const int tid = threadIdx.x;
+    const uint32_t d = blockIdx.y;
+    const double sg = sigma[d];
+    if (!(sg > 0.0)) return;
+    const double mu = mean[d];
+    const float *row = x + (int64_t)d * stride;
+    const int64_t b = (int64_t)blockIdx.x * kBT - 1;     // sample at local index 0
+    const int L = kBT + (int)maxw + 1;
+    for (int u = tid; u < L; u += 256) {
+        const int64_t t = b + u;
+        buf[0][u] = (t >= 0 && t < (int64_t)n) ? (double)row[t] : 0.0;
+    }
+    __syncthreads();
+    int cur = 0;
+    for (uint32_t w = 1, wi = 0; w <= maxw; w <<= 1, ++wi) {
+        if (wmask & w) {
+            const double wm = (double)w * mu;
+            const double den = sg * sqrtw.v[wi];
+            const int64_t last = (int64_t)n - w;
+            for (int u = 1 + tid; u <= kBT; u += 256) {
+                const int64_t t = b + u;
+                if (t > last) break;
+                const double sv = buf[cur][u];
+                const double num = sv - wm;
+                const double snr = num / den;
+                if (!(snr > thr)) continue;
+                if (t > 0 && !(sv >= buf[cur][u - 1])) continue;
+                if (t < last && !(sv > buf[cur][u + 1])) continue;
+                const uint32_t slot = atomicAdd(count, 1u);
+                if (slot < cap) {
+                    mdsm_cand cd;
+                    cd.dm = d;
+                    cd.t = (uint32_t)t;
+                    cd.width = w;
+                    cd.snr = (float)snr;
+                    cands[slot] = cd;
+                }
+            }
+        }
+        if (2 * w > maxw) break;
+        // next level: S_2w[u] = S_w[u] + S_w[u+w], u <= L - 2w
+        const int lim = L - 2 * (int)w;
+        for (int u = tid; u <= lim; u += 256) buf[cur ^ 1][u] = buf[cur][u] + buf[cur][u + w];
+        __syncthreads();
+        cur ^= 1;
+    }
+}
+
+hipError_t launch_boxcar(const float *x, uint32_t ndms, uint32_t n, int64_t stride, uint32_t wmask,
+                         double thr, const SqrtW &sqrtw, mdsm_cand *cands, uint32_t cap,
+                         uint32_t *count, double *mean, double *sigma, hipStream_t s)
+{
+    if (ndms == 0 || n == 0) return hipSuccess;
+    uint32_t maxw = 1;
+    while ((maxw << 1) <= wmask && (maxw << 1) <= (uint32_t)kMaxBoxW) maxw <<= 1;
+    hipLaunchKernelGGL(row_sums_kernel, dim3(ndms), dim3(256), 0, s, x, n, stride, mean, sigma);
+    hipLaunchKernelGGL(row_stats_finish, dim3((ndms + 255) / 256), dim3(256), 0, s, ndms, n, mean,
+                       sigma);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, count, 1u, 0u);
+    dim3 grid((n + kBT - 1) / kBT, ndms);
+    hipLaunchKernelGGL(boxcar_kernel, grid, dim3(256), 0, s, x, n, stride, wmask, maxw, thr, sqrtw,
+                       cands, cap, count, mean, sigma);
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Median-of-five threshold detector (compat with the reference output thread)
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ float med5(float a, float b, float c, float d, float e)
+{
+    // sorting network on 5 values; the middle element is the median
+    float t;
+#define CSWAP(p, q) \
+    if (q < p) {    \
+        t = p;      \
+        p = q;      \
+        q = t;      \
+    }
+    CSWAP(a, b);
+    CSWAP(d, e);
+    CSWAP(c, e);
+    CSWAP(c, d);
+    CSWAP(b, e);
+    CSWAP(a, d);
+    CSWAP(a, c);
+    CSWAP(b, d);
+    CSWAP(b, c);
+#undef CSWAP
+    return c;
+}
+
+__global__ void block_stats_finish(const double *s1, const double *s2, uint32_t ndms, uint32_t n,
+                                   double *stats)
+{
+    // single wave: ordered partial sums over rows, then a fixed tree
+    const int lane = threadIdx.x;
+    double a = 0.0, b = 0.0;
+    for (uint32_t d = lane; d < ndms; d += 64) {
+        a += s1[d];
+        b += s2[d];
+    }
+    a = wave_sum_f64(a);
+    b = wave_sum_f64(b);
+    if (lane == 0) {
+        const double cnt = (double)ndms * (double)n;
+        const double m = a / cnt;
+        const double ex2 = b / cnt;
+        const double mm = m * m;
+        double var = ex2 - mm;
+        if (var < 0.0) var = 0.0;
+        stats[0] = m;
+        stats[1] = sqrt(var);
+    }
+}
+
+__global__ void __launch_bounds__(256)
+    median5_kernel(const float *x, uint32_t n, int64_t stride, const float *dmv, float thr,
+                   mdsm_cand *cands, uint32_t cap, uint32_t *count, const double *stats)
+{
+    const uint32_t d = blockIdx.y;
+    const uint32_t l = blockIdx.x * 256u + threadIdx.x;
+    if (n < 5 || l >= n - 4) return;
+    if (!(dmv[d] > 1.0f)) return;
+    const float lm = (float)stats[0], lr = (float)stats[1];
+    const float *r = x + (int64_t)d * stride + l;
+    const float m = med5(r[0], r[1], r[2], r[3], r[4]);
+    if (m - lm >= lr * thr) {
+        const uint32_t slot = atomicAdd(count, 1u);
+        if (slot < cap) {
+            mdsm_cand cd;
+            cd.dm = d;
+            cd.t = l;
+            cd.width = 5;
+            cd.snr = m / lr;
+            cands[slot] = cd;
+        }
+    }
+}
+
+hipError_t launch_median5(const float *x, uint32_t ndms, uint32_t n, int64_t stride,
+                          const float *dmv, double thr, mdsm_cand *cands, uint32_t cap,
+                          uint32_t *count, double *stats, hipStream_t s)
+{
+    if (ndms == 0 || n < 5) return hipSuccess;
+    // stats scratch: stats[2 .. 2+2*ndms) holds the per-row sums
+    double *s1 = stats + 2, *s2 = stats + 2 + ndms;
+    hipLaunchKernelGGL(row_sums_kernel, dim3(ndms), dim3(256), 0, s, x, n, stride, s1, s2);
+    hipLaunchKernelGGL(block_stats_finish, dim3(1), dim3(64), 0, s, s1, s2, ndms, n, stats);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, count, 1u, 0u);
+    dim3 grid((n - 4 + 255) / 256, ndms);
+    hipLaunchKernelGGL(median5_kernel, grid, dim3(256), 0, s, x, n, stride, dmv, (float)thr, cands,
+                       cap, count, stats);
+    return hipGetLastError();
+}
+
+}  // namespace mdsm

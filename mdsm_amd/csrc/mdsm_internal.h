@@ -1,0 +1,160 @@
+// Internal structures shared by the host plan builder and the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime_api.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mdsm_c.h"
+
+namespace mdsm {
+
+// Dedispersion tiling (the kernel's compile-time shape).
+constexpr int kWaves = 4;            // waves per workgroup
+constexpr int kRPW = 8;              // DM trials per wave (registers)
+constexpr int kDB = kWaves * kRPW;   // DM trials per block
+constexpr int kPPL = 4;              // output samples per lane (stride 64)
+constexpr int kTile = 64 * kPPL;     // output samples per tile
+constexpr int kCC = 16;              // channels per LDS chunk
+constexpr int kMaxW = 1024;          // widest LDS row (floats)
+
+// Buffers a block can read or write.
+enum BufId : int32_t { BUF_IN = 0, BUF_OUT = 1, BUF_WORK = 2 };
+
+// Internal sample types (the public three plus uint16 decimated sums).
+enum DType : int32_t { DT_U8 = 0, DT_F16 = 1, DT_F32 = 2, DT_U16 = 3 };
+
+inline size_t dtype_size(int dt)
+{
+    return dt == DT_U8 ? 1 : (dt == DT_F32 ? 4 : 2);
+}
+
+// One block of up to kDB trials that share one set of input rows.
+// Trial j of the block computes, for t < nout,
+//   out[out_off + j*out_trial_stride + t] =
+//       (1/k) * sum_{i<nrows} in[row0+i][abase[i] + t + rel[i][j]]
+// where in[row][col] = buf[src][row*row_stride + col] (row_stride of BUF_IN
+// is the caller's in_stride, given at launch).
+struct BlockDesc {
+    int64_t row0;
+    int64_t row_stride;
+    int64_t in_off;           // element offset of row 0 col 0 within buf[src]
+    int64_t out_off;          // float offset of trial 0 within buf[dst]
+    int64_t out_trial_stride;
+    int32_t in_len;           // valid columns per row
+    int32_t nrows;
+    int32_t ntrials;
+    int32_t W;                // LDS row width (floats), multiple of 64
+    int32_t nout;
+    int32_t k;                // divisor applied at store (1 = none)
+    int32_t src, dst;
+    int32_t dtype;
+    int32_t tab_off;          // int offset of rel[nrows][kDB] in the rel table
+    int32_t base_off;         // int offset of abase[nrows] in the base table
+    int32_t pad;
+};
+
+// A run of blocks tiled over the same number of output tiles; flat work
+// items of a segment are ordered tile-major (all blocks of tile 0 first).
+struct SegDesc {
+    int64_t item_start;
+    int32_t first_block;
+    int32_t nblocks;
+    int32_t ntiles;
+    int32_t pad;
+};
+
+// One dedispersion launch: a contiguous run of segments.
+struct Stage {
+    int32_t seg_begin = 0, seg_end = 0;
+    int64_t items = 0;
+    int32_t wmax = 0;
+};
+
+struct DecimStep {
+    int32_t k = 1;
+    int32_t dtype_in = DT_U8, dtype_out = DT_U8;
+    int64_t work_off = 0;     // byte offset of the decimated copy in workspace
+    int64_t stride = 0;       // elements per row of the copy
+    int32_t nsamp_out = 0;
+};
+
+struct RangeHost {
+    mdsm_range_info info{};
+    std::vector<int32_t> shift;   // [ndms][nchans] brute; [ncalls][nchans] subband coarse
+    std::vector<int32_t> fine;    // subband stage-2 [ncalls*calldms][nsubs]
+    std::vector<float> dm_values;
+    int32_t decim = -1;           // DecimStep index, -1 = raw input
+    mdsm_pass pass{};
+    int64_t sb_off = 0;           // subband stage-1 buffer (byte offset)
+    int32_t len1 = 0;
+};
+
+struct KernelTables {
+    BlockDesc *blocks = nullptr;
+    SegDesc *segs = nullptr;
+    int32_t *rel = nullptr;
+    int32_t *abase = nullptr;
+};
+
+}  // namespace mdsm
+
+struct mdsm_plan {
+    mdsm_obs obs{};
+    int device = 0;
+    bool uploaded = false;  // device tables are uploaded on first use
+    bool subband = false;
+    uint32_t nsubs = 0;
+    std::vector<mdsm::RangeHost> ranges;
+    std::vector<mdsm::DecimStep> decim;
+    std::vector<mdsm::BlockDesc> blocks;
+    std::vector<mdsm::SegDesc> segs;
+    std::vector<int32_t> rel, abase;
+    std::vector<mdsm::Stage> stages;   // launched in order
+    mdsm::KernelTables dev{};
+    void *own_work = nullptr;
+    uint64_t work_bytes = 0;
+    mdsm_plan_info info{};
+};
+
+namespace mdsm {
+
+int upload_plan(mdsm_plan *p);
+
+// Error plumbing (capi.cpp).
+int fail(int code, const std::string &msg);
+int hip_fail(hipError_t e, const char *what);
+
+// Timing hooks around launches (capi.cpp).
+struct TimedLaunch {
+    TimedLaunch(const char *name, hipStream_t s);
+    ~TimedLaunch();
+    const char *name;
+    hipStream_t stream;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+};
+
+// Launchers (kernels.hip).
+hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in,
+                             int64_t in_stride, float *out, void *work, hipStream_t s);
+hipError_t launch_decimate(const void *in, int dtype_in, uint32_t nchans, int64_t in_stride,
+                           uint32_t nsamp, uint32_t k, void *out, int dtype_out,
+                           int64_t out_stride, hipStream_t s);
+hipError_t launch_clip_u8(const uint8_t *in, int64_t in_stride, uint32_t nchans, int64_t t0,
+                          uint32_t n, float nsigma, uint8_t *out, int64_t out_stride,
+                          int32_t *med, int32_t *mad, int32_t *ncl, hipStream_t s);
+struct SqrtW {
+    double v[11];  // sqrt(2^i), i = 0..10, computed on the host
+};
+hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
+hipError_t launch_boxcar(const float *x, uint32_t ndms, uint32_t n, int64_t stride,
+                         uint32_t wmask, double thr, const SqrtW &sqrtw, mdsm_cand *cands,
+                         uint32_t cap, uint32_t *count, double *mean, double *sigma,
+                         hipStream_t s);
+hipError_t launch_median5(const float *x, uint32_t ndms, uint32_t n, int64_t stride,
+                          const float *dmv, double thr, mdsm_cand *cands, uint32_t cap,
+                          uint32_t *count, double *stats, hipStream_t s);
+
+}  // namespace mdsm

@@ -1,0 +1,629 @@
+// Host-side plan builder: shift tables, output layout, device work tables.
+//
+// Shift tables are integer tables built once per plan on the host, so the
+// rounding of the delay arithmetic is decided here, in one place, and every
+// kernel reads plain int32 offsets.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <functional>
+#include <limits>
+
+#include "mdsm_internal.h"
+
+namespace mdsm {
+namespace {
+
+inline int64_t round_up(int64_t v, int64_t m) { return (v + m - 1) / m * m; }
+
+// Delay model of one observation under one shift rule.
+struct ShiftModel {
+    int rule = MDSM_SHIFT_REF;
+    std::vector<float> delay_f;   // REF: seconds per unit DM, float32
+    std::vector<double> delay_d;  // NS: f^-2 - f_top^-2 (MHz^-2)
+
+    ShiftModel(const mdsm_obs &o)
+    {
+        rule = o.shift_rule;
+        delay_f.resize(o.nchans);
+        delay_d.resize(o.nchans);
+        for (uint32_t c = 0; c < o.nchans; ++c) {
+            // REF: channel frequency formed in float, delay evaluated in double
+            // and narrowed to float (src/mdsm/dedispersion_manager.cpp:262-265,
+            // 287-289; src/prototypes/cpu_dedisp.c:52-55).
+            float fc = o.foff * (float)c;
+            fc = o.fch1 + fc;
+            double a = 1.0 / (double)fc;
+            a = a / (double)fc;
+            double b = 1.0 / (double)o.fch1;
+            b = b / (double)o.fch1;
+            delay_f[c] = (float)(4148.741601 * (a - b));
+            // NS: FP64 physical delay, constant 4.148808e3 s MHz^2.
+            double fd = (double)o.fch1 + (double)o.foff * (double)c;
+            delay_d[c] = 1.0 / (fd * fd) - 1.0 / ((double)o.fch1 * (double)o.fch1);
+        }
+    }
+
+    // REF: (int)(delay * dm / tsamp) in float32, left to right, truncated
+    // (src/prototypes/cpu_dedisp.c:72).  NS: nearest sample in FP64.
+    int32_t shift(uint32_t c, float dm_f, double dm_d, float tsamp_f, double tsamp_d) const
+    {
+        if (rule == MDSM_SHIFT_REF) {
+            float v = delay_f[c] * dm_f;
+            v = v / tsamp_f;
+            return (int32_t)v;
+        }
+        double v = 4.148808e3 * dm_d * delay_d[c];
+        return (int32_t)std::llround(v / tsamp_d);
+    }
+};
+
+int validate_obs(const mdsm_obs *o)
+{
+    if (!o) return fail(MDSM_ERR_INVALID, "obs is NULL");
+    if (o->nchans < 1 || o->nchans > (1u << 20))
+        return fail(MDSM_ERR_INVALID, "nchans out of range");
+    if (o->nsamp_in < 1) return fail(MDSM_ERR_INVALID, "nsamp_in must be >= 1");
+    if (!(o->tsamp > 0.0f) || !std::isfinite(o->tsamp))
+        return fail(MDSM_ERR_INVALID, "tsamp must be > 0");
+    if (!(o->fch1 > 0.0f) || !std::isfinite(o->fch1))
+        return fail(MDSM_ERR_INVALID, "fch1 must be > 0");
+    if (!(o->foff < 0.0f))
+        return fail(MDSM_ERR_INVALID, "foff must be negative (channel 0 is the top of the band)");
+    float flow = o->fch1 + o->foff * (float)(o->nchans - 1);
+    if (!(flow > 0.0f)) return fail(MDSM_ERR_INVALID, "lowest channel frequency must be > 0");
+    if (o->dtype < MDSM_U8 || o->dtype > MDSM_F32) return fail(MDSM_ERR_INVALID, "bad dtype");
+    if (o->shift_rule != MDSM_SHIFT_REF && o->shift_rule != MDSM_SHIFT_NS)
+        return fail(MDSM_ERR_INVALID, "bad shift rule");
+    return MDSM_OK;
+}
+
+// Appends one segment: trials [0, ntrials) over rows [0, nrows), split into
+// blocks of at most kDB trials whose LDS window fits kMaxW.
+struct SegSpec {
+    int ntrials, nrows;
+    std::function<int32_t(int, int)> sh;  // (trial, row) -> shift >= 0
+    int src;
+    int64_t row0, row_stride, in_off;
+    int in_len, dtype;
+    int dst;
+    int64_t out_off, out_trial_stride;
+    int nout, k;
+};
+
+int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
+{
+    SegDesc seg{};
+    seg.first_block = (int32_t)p->blocks.size();
+    seg.ntiles = (int32_t)((s.nout + kTile - 1) / kTile);
+    std::vector<int32_t> base(s.nrows);
+    int j0 = 0;
+    while (j0 < s.ntrials) {
+        int nt = std::min(kDB, s.ntrials - j0);
+        int64_t relmax = 0;
+        for (;;) {
+            relmax = 0;
+            for (int i = 0; i < s.nrows; ++i) {
+                int32_t lo = std::numeric_limits<int32_t>::max(), hi = 0;
+                for (int j = 0; j < nt; ++j) {
+                    int32_t v = s.sh(j0 + j, i);
+                    lo = std::min(lo, v);
+                    hi = std::max(hi, v);
+                }
+                if (lo < 0) return fail(MDSM_ERR_INVALID, "negative shift in plan");
+                base[i] = lo & ~15;
+                relmax = std::max<int64_t>(relmax, hi - base[i]);
+            }
+            if (round_up(kTile + relmax, 64) <= kMaxW || nt == 1) break;
+            nt = (nt + 1) / 2;
+        }
+        BlockDesc b{};
+        b.row0 = s.row0;
+        b.row_stride = s.row_stride;
+        b.in_off = s.in_off;
+        b.out_off = s.out_off + (int64_t)j0 * s.out_trial_stride;
+        b.out_trial_stride = s.out_trial_stride;
+        b.in_len = s.in_len;
+        b.nrows = s.nrows;
+        b.ntrials = nt;
+        b.W = (int32_t)round_up(kTile + relmax, 64);
+        b.nout = s.nout;
+        b.k = s.k;
+        b.src = s.src;
+        b.dst = s.dst;
+        b.dtype = s.dtype;
+        b.tab_off = (int32_t)p->rel.size();
+        b.base_off = (int32_t)p->abase.size();
+        if (p->rel.size() + (size_t)s.nrows * kDB > (size_t)std::numeric_limits<int32_t>::max())
+            return fail(MDSM_ERR_INVALID, "plan shift tables exceed 2^31 entries");
+        for (int i = 0; i < s.nrows; ++i) {
+            p->abase.push_back(base[i]);
+            for (int j = 0; j < kDB; ++j)
+                p->rel.push_back(j < nt ? s.sh(j0 + j, i) - base[i] : 0);
+        }
+        st.wmax = std::max(st.wmax, b.W);
+        p->blocks.push_back(b);
+        j0 += nt;
+    }
+    seg.nblocks = (int32_t)p->blocks.size() - seg.first_block;
+    seg.item_start = st.items;
+    st.items += (int64_t)seg.nblocks * seg.ntiles;
+    p->segs.push_back(seg);
+    st.seg_end = (int32_t)p->segs.size();
+    return MDSM_OK;
+}
+
+int add_decimation(mdsm_plan *p, int k, uint64_t &work)
+{
+    for (size_t i = 0; i < p->decim.size(); ++i)
+        if (p->decim[i].k == k) return (int)i;
+    DecimStep d;
+    d.k = k;
+    d.dtype_in = p->obs.dtype;
+    d.dtype_out = p->obs.dtype == MDSM_U8 ? DT_U16 : DT_F32;
+    d.nsamp_out = (int32_t)(p->obs.nsamp_in / (uint32_t)k);
+    d.stride = round_up(d.nsamp_out, 64);
+    d.work_off = (int64_t)round_up((int64_t)work, 256);
+    work = d.work_off + (uint64_t)p->obs.nchans * d.stride * dtype_size(d.dtype_out);
+    p->decim.push_back(d);
+    return (int)p->decim.size() - 1;
+}
+
+int upload(mdsm_plan *p)
+{
+    hipError_t e = hipGetDevice(&p->device);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    p->uploaded = true;
+    auto up = [&](void **dst, const void *src, size_t bytes) -> hipError_t {
+        hipError_t r = hipMalloc(dst, bytes ? bytes : 16);
+        if (r != hipSuccess) return r;
+        if (src && bytes) return hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+        return hipSuccess;
+    };
+    if ((e = up((void **)&p->dev.blocks, p->blocks.data(), p->blocks.size() * sizeof(BlockDesc))) !=
+            hipSuccess ||
+        (e = up((void **)&p->dev.segs, p->segs.data(), p->segs.size() * sizeof(SegDesc))) !=
+            hipSuccess ||
+        (e = up((void **)&p->dev.rel, p->rel.data(), p->rel.size() * sizeof(int32_t))) !=
+            hipSuccess ||
+        (e = up((void **)&p->dev.abase, p->abase.data(), p->abase.size() * sizeof(int32_t))) !=
+            hipSuccess)
+        return hip_fail(e, "uploading plan tables");
+    return MDSM_OK;
+}
+
+void finish_info(mdsm_plan *p, double adds, double bytes, double dcs)
+{
+    mdsm_plan_info &in = p->info;
+    in.nranges = (uint32_t)p->ranges.size();
+    in.nchans = p->obs.nchans;
+    in.work_bytes = p->work_bytes;
+    in.adds = adds;
+    in.bytes = bytes;
+    in.dm_chan_samples = dcs;
+    in.is_subband = p->subband ? 1u : 0u;
+    in.nblocks = (uint32_t)p->blocks.size();
+}
+
+}  // namespace
+
+int upload_plan(mdsm_plan *p) { return p->uploaded ? MDSM_OK : upload(p); }
+
+}  // namespace mdsm
+
+using namespace mdsm;
+
+extern "C" int mdsm_plan_create(const mdsm_obs *obs, const mdsm_range *ranges, uint32_t nranges,
+                                mdsm_plan **out)
+{
+    if (!out) return fail(MDSM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int rc = validate_obs(obs);
+    if (rc) return rc;
+    if (!ranges || nranges < 1 || nranges > 64)
+        return fail(MDSM_ERR_INVALID, "need 1..64 DM ranges");
+    auto *p = new mdsm_plan();
+    p->obs = *obs;
+    ShiftModel model(*obs);
+    const uint32_t C = obs->nchans, S = obs->nsamp_in;
+    uint64_t work = 0, out_elems = 0;
+    uint32_t dm_row = 0, maxshift = 0;
+    double adds = 0, bytes = (double)C * S * dtype_size(obs->dtype), dcs = 0;
+    for (uint32_t r = 0; r < nranges; ++r) {
+        const mdsm_range &R = ranges[r];
+        if (R.ndms < 1 || R.downsample < 1 || R.downsample > 256 || !(R.lowdm >= 0.0f) ||
+            !(R.dmstep >= 0.0f) || !std::isfinite(R.lowdm) || !std::isfinite(R.dmstep)) {
+            delete p;
+            return fail(MDSM_ERR_INVALID, "range " + std::to_string(r) + " is malformed");
+        }
+        RangeHost rh;
+        const int k = (int)R.downsample;
+        const uint32_t Sk = S / (uint32_t)k;
+        const float tsf = obs->tsamp * (float)k;
+        const double tsd = (double)obs->tsamp * (double)k;
+        rh.shift.resize((size_t)R.ndms * C);
+        rh.dm_values.resize(R.ndms);
+        int32_t M = 0;
+        for (uint32_t n = 0; n < R.ndms; ++n) {
+            float dmf = (float)n * R.dmstep;
+            dmf = R.lowdm + dmf;
+            double dmd = (double)R.lowdm + (double)n * (double)R.dmstep;
+            rh.dm_values[n] = dmf;
+            for (uint32_t c = 0; c < C; ++c) {
+                int32_t v = model.shift(c, dmf, dmd, tsf, tsd);
+                rh.shift[(size_t)n * C + c] = v;
+                M = std::max(M, v);
+            }
+        }
+        if ((int64_t)Sk - M < 1) {
+            delete p;
+            return fail(MDSM_ERR_INVALID, "range " + std::to_string(r) +
+                                              ": block too short for its maximum delay (" +
+                                              std::to_string(M) + " samples at tsamp*k)");
+        }
+        mdsm_range_info &ri = rh.info;
+        ri.lowdm = R.lowdm;
+        ri.dmstep = R.dmstep;
+        ri.ndms = R.ndms;
+        ri.downsample = R.downsample;
+        ri.maxshift = (uint32_t)M;
+        ri.nout = Sk - (uint32_t)M;
+        ri.out_offset = out_elems;
+        ri.first_dm = dm_row;
+        ri.ncalls = R.ndms;
+        out_elems += (uint64_t)R.ndms * ri.nout;
+        dm_row += R.ndms;
+        maxshift = std::max<uint32_t>(maxshift, (uint32_t)k * (uint32_t)M);
+        if (k > 1) rh.decim = add_decimation(p, k, work);
+        adds += (double)R.ndms * C * ri.nout;
+        dcs += (double)R.ndms * C * ri.nout;
+        bytes += (double)R.ndms * ri.nout * 4.0;
+        p->ranges.push_back(std::move(rh));
+    }
+    for (auto &d : p->decim) {
+        adds += (double)C * d.nsamp_out * (d.k - 1);
+        bytes += (double)C * d.nsamp_out * dtype_size(d.dtype_out) * 2.0;  // written, read back
+    }
+    p->work_bytes = work;
+    // One launch covers every range.
+    Stage st;
+    st.seg_begin = 0;
+    for (uint32_t r = 0; r < nranges; ++r) {
+        RangeHost &rh = p->ranges[r];
+        const int32_t *sh = rh.shift.data();
+        SegSpec s;
+        s.ntrials = (int)rh.info.ndms;
+        s.nrows = (int)C;
+        s.sh = [sh, C](int j, int i) { return sh[(size_t)j * C + i]; };
+        if (rh.decim < 0) {
+            s.src = BUF_IN;
+            s.row_stride = 0;  // caller's in_stride
+            s.in_off = 0;
+            s.in_len = (int)S;
+            s.dtype = obs->dtype;
+        } else {
+            const DecimStep &d = p->decim[rh.decim];
+            s.src = BUF_WORK;
+            s.row_stride = d.stride;
+            s.in_off = d.work_off / (int64_t)dtype_size(d.dtype_out);
+            s.in_len = d.nsamp_out;
+            s.dtype = d.dtype_out;
+        }
+        s.row0 = 0;
+        s.dst = BUF_OUT;
+        s.out_off = (int64_t)rh.info.out_offset;
+        s.out_trial_stride = rh.info.nout;
+        s.nout = (int)rh.info.nout;
+        s.k = (int)rh.info.downsample;
+        int rc2 = add_segment(p, st, s);
+        if (rc2) {
+            delete p;
+            return rc2;
+        }
+    }
+    p->stages.push_back(st);
+    p->info.ndms_total = dm_row;
+    p->info.maxshift = maxshift;
+    p->info.out_elems = out_elems;
+    bytes += (double)p->rel.size() * 4.0;
+    finish_info(p, adds, bytes, dcs);
+    *out = p;
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_plan_create_subband(const mdsm_obs *obs, uint32_t nsubs, const mdsm_pass *passes,
+                                        uint32_t npasses, mdsm_plan **out)
+{
+    if (!out) return fail(MDSM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    int rc = validate_obs(obs);
+    if (rc) return rc;
+    const uint32_t C = obs->nchans, S = obs->nsamp_in;
+    if (nsubs < 1 || C % nsubs != 0)
+        return fail(MDSM_ERR_INVALID, "nsubs must divide nchans");
+    if (!passes || npasses < 1 || npasses > 64)
+        return fail(MDSM_ERR_INVALID, "need 1..64 subband passes");
+    const uint32_t cps = C / nsubs;
+    auto *p = new mdsm_plan();
+    p->obs = *obs;
+    p->subband = true;
+    p->nsubs = nsubs;
+    ShiftModel model(*obs);
+    uint64_t work = 0, out_elems = 0;
+    uint32_t dm_row = 0, maxshift = 0;
+    double adds = 0, bytes = (double)C * S * dtype_size(obs->dtype), dcs = 0;
+    for (uint32_t q = 0; q < npasses; ++q) {
+        const mdsm_pass &P = passes[q];
+        if (P.binsize < 1 || P.binsize > 256 || P.ncalls < 1 || P.calldms < 1 ||
+            P.ndms != P.ncalls * P.calldms || !(P.lowdm >= 0.0f) || !(P.dmstep >= 0.0f) ||
+            !(P.sub_dmstep >= 0.0f)) {
+            delete p;
+            return fail(MDSM_ERR_INVALID, "pass " + std::to_string(q) +
+                                              " is malformed (ndms must equal ncalls*calldms)");
+        }
+        RangeHost rh;
+        rh.pass = P;
+        const int k = (int)P.binsize;
+        const uint32_t Sk = S / (uint32_t)k;
+        const float tsf = obs->tsamp * (float)k;
+        const double tsd = (double)obs->tsamp * (double)k;
+        rh.shift.resize((size_t)P.ncalls * C);
+        rh.fine.resize((size_t)P.ndms * nsubs);
+        rh.dm_values.resize(P.ndms);
+        int32_t M1 = 0, M2 = 0;
+        for (uint32_t j = 0; j < P.ncalls; ++j) {
+            // Coarse DM of call j (src/mdsm/dedispersion_thread.cu:161-168).
+            float cdm = (float)j * P.sub_dmstep;
+            cdm = P.lowdm + cdm;
+            double cdmd = (double)P.lowdm + (double)j * (double)P.sub_dmstep;
+            int32_t *co = &rh.shift[(size_t)j * C];
+            for (uint32_t c = 0; c < C; ++c) co[c] = model.shift(c, cdm, cdmd, tsf, tsd);
+            for (uint32_t c = 0; c < C; ++c) {
+                int32_t rel = co[c] - co[(c / cps) * cps];
+                if (rel < 0) {
+                    delete p;
+                    return fail(MDSM_ERR_INVALID, "negative subband-relative shift");
+                }
+                M1 = std::max(M1, rel);
+            }
+            // Fine DMs of call j centred on the coarse DM
+            // (src/mdsm/dedispersion_thread.cu:195-205).
+            float half = (float)P.calldms * P.dmstep;
+            half = half / 2.0f;
+            float start = cdm - half;
+            if (!(start > 0.0f)) start = 0.0f;
+            double startd = std::max(cdmd - (double)P.calldms * (double)P.dmstep / 2.0, 0.0);
+            for (uint32_t i = 0; i < P.calldms; ++i) {
+                float fdm = (float)i * P.dmstep;
+                fdm = start + fdm;
+                double fdmd = startd + (double)i * (double)P.dmstep;
+                size_t d = (size_t)j * P.calldms + i;
+                rh.dm_values[d] = fdm;
+                for (uint32_t b = 0; b < nsubs; ++b) {
+                    int32_t v = model.shift(b * cps, fdm, fdmd, tsf, tsd);
+                    rh.fine[d * nsubs + b] = v;
+                    M2 = std::max(M2, v);
+                }
+            }
+        }
+        int64_t N = (int64_t)Sk - M1 - M2;
+        if (N < 1) {
+            delete p;
+            return fail(MDSM_ERR_INVALID, "pass " + std::to_string(q) +
+                                              ": block too short for its maximum delay");
+        }
+        rh.len1 = (int32_t)(N + M2);
+        mdsm_range_info &ri = rh.info;
+        ri.lowdm = P.lowdm;
+        ri.dmstep = P.dmstep;
+        ri.ndms = P.ndms;
+        ri.downsample = P.binsize;
+        ri.maxshift = (uint32_t)(M1 + M2);
+        ri.nout = (uint32_t)N;
+        ri.out_offset = out_elems;
+        ri.first_dm = dm_row;
+        ri.ncalls = P.ncalls;
+        out_elems += (uint64_t)P.ndms * ri.nout;
+        dm_row += P.ndms;
+        maxshift = std::max<uint32_t>(maxshift, (uint32_t)k * (uint32_t)(M1 + M2));
+        if (k > 1) rh.decim = add_decimation(p, k, work);
+        work = (uint64_t)round_up((int64_t)work, 256);
+        rh.sb_off = (int64_t)work;
+        work += (uint64_t)P.ncalls * nsubs * (uint64_t)rh.len1 * 4u;
+        adds += (double)P.ncalls * C * rh.len1 + (double)P.ndms * nsubs * N;
+        dcs += (double)P.ndms * C * N;
+        bytes += (double)P.ncalls * nsubs * rh.len1 * 4.0 * 2.0 + (double)P.ndms * N * 4.0;
+        p->ranges.push_back(std::move(rh));
+    }
+    for (auto &d : p->decim) {
+        adds += (double)C * d.nsamp_out * (d.k - 1);
+        bytes += (double)C * d.nsamp_out * dtype_size(d.dtype_out) * 2.0;
+    }
+    p->work_bytes = work;
+    // Stage 1: per pass and subband, coarse DMs over the subband's channels.
+    Stage s1;
+    s1.seg_begin = 0;
+    for (auto &rh : p->ranges) {
+        const int32_t *co = rh.shift.data();
+        for (uint32_t b = 0; b < nsubs; ++b) {
+            SegSpec s;
+            s.ntrials = (int)rh.pass.ncalls;
+            s.nrows = (int)cps;
+            const uint32_t top = b * cps;
+            s.sh = [co, C, top](int j, int i) {
+                const int32_t *row = co + (size_t)j * C;
+                return row[top + i] - row[top];
+            };
+            if (rh.decim < 0) {
+                s.src = BUF_IN;
+                s.row_stride = 0;
+                s.in_off = 0;
+                s.in_len = (int)S;
+                s.dtype = obs->dtype;
+            } else {
+                const DecimStep &d = p->decim[rh.decim];
+                s.src = BUF_WORK;
+                s.row_stride = d.stride;
+                s.in_off = d.work_off / (int64_t)dtype_size(d.dtype_out);
+                s.in_len = d.nsamp_out;
+                s.dtype = d.dtype_out;
+            }
+            s.row0 = top;
+            s.dst = BUF_WORK;
+            s.out_off = rh.sb_off / 4 + (int64_t)b * rh.len1;
+            s.out_trial_stride = (int64_t)nsubs * rh.len1;
+            s.nout = rh.len1;
+            s.k = 1;
+            int rc2 = add_segment(p, s1, s);
+            if (rc2) {
+                delete p;
+                return rc2;
+            }
+        }
+    }
+    // Stage 2: per pass and call, fine DMs over the call's subband series.
+    Stage s2;
+    s2.seg_begin = (int32_t)p->segs.size();
+    for (auto &rh : p->ranges) {
+        const int32_t *fi = rh.fine.data();
+        for (uint32_t j = 0; j < rh.pass.ncalls; ++j) {
+            SegSpec s;
+            s.ntrials = (int)rh.pass.calldms;
+            s.nrows = (int)nsubs;
+            const size_t d0 = (size_t)j * rh.pass.calldms;
+            const uint32_t ns = nsubs;
+            s.sh = [fi, d0, ns](int t, int b) { return fi[(d0 + t) * ns + b]; };
+            s.src = BUF_WORK;
+            s.row0 = 0;
+            s.row_stride = rh.len1;
+            s.in_off = rh.sb_off / 4 + (int64_t)j * nsubs * rh.len1;
+            s.in_len = rh.len1;
+            s.dtype = DT_F32;
+            s.dst = BUF_OUT;
+            s.out_off = (int64_t)rh.info.out_offset + (int64_t)d0 * rh.info.nout;
+            s.out_trial_stride = rh.info.nout;
+            s.nout = (int)rh.info.nout;
+            s.k = (int)rh.pass.binsize;
+            int rc2 = add_segment(p, s2, s);
+            if (rc2) {
+                delete p;
+                return rc2;
+            }
+        }
+    }
+    p->stages.push_back(s1);
+    p->stages.push_back(s2);
+    p->info.ndms_total = dm_row;
+    p->info.maxshift = maxshift;
+    p->info.out_elems = out_elems;
+    bytes += (double)p->rel.size() * 4.0;
+    finish_info(p, adds, bytes, dcs);
+    *out = p;
+    return MDSM_OK;
+}
+
+extern "C" void mdsm_plan_destroy(mdsm_plan *p)
+{
+    if (!p) return;
+    if (!p->uploaded) {
+        delete p;
+        return;
+    }
+    int cur = 0;
+    bool sw = hipGetDevice(&cur) == hipSuccess && cur != p->device;
+    if (sw) (void)hipSetDevice(p->device);
+    if (p->dev.blocks) (void)hipFree(p->dev.blocks);
+    if (p->dev.segs) (void)hipFree(p->dev.segs);
+    if (p->dev.rel) (void)hipFree(p->dev.rel);
+    if (p->dev.abase) (void)hipFree(p->dev.abase);
+    if (p->own_work) (void)hipFree(p->own_work);
+    if (sw) (void)hipSetDevice(cur);
+    delete p;
+}
+
+extern "C" int mdsm_plan_get_info(const mdsm_plan *p, mdsm_plan_info *info)
+{
+    if (!p || !info) return fail(MDSM_ERR_INVALID, "NULL argument");
+    *info = p->info;
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_plan_get_range(const mdsm_plan *p, uint32_t r, mdsm_range_info *info)
+{
+    if (!p || !info) return fail(MDSM_ERR_INVALID, "NULL argument");
+    if (r >= p->ranges.size()) return fail(MDSM_ERR_INVALID, "range index out of bounds");
+    *info = p->ranges[r].info;
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_plan_shift_table(const mdsm_plan *p, uint32_t r, int32_t *host_out)
+{
+    if (!p || !host_out) return fail(MDSM_ERR_INVALID, "NULL argument");
+    if (r >= p->ranges.size()) return fail(MDSM_ERR_INVALID, "range index out of bounds");
+    const auto &v = p->ranges[r].shift;
+    std::memcpy(host_out, v.data(), v.size() * sizeof(int32_t));
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_plan_fine_table(const mdsm_plan *p, uint32_t r, int32_t *host_out)
+{
+    if (!p || !host_out) return fail(MDSM_ERR_INVALID, "NULL argument");
+    if (!p->subband) return fail(MDSM_ERR_INVALID, "not a subband plan");
+    if (r >= p->ranges.size()) return fail(MDSM_ERR_INVALID, "pass index out of bounds");
+    const auto &v = p->ranges[r].fine;
+    std::memcpy(host_out, v.data(), v.size() * sizeof(int32_t));
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_plan_dm_values(const mdsm_plan *p, float *host_out)
+{
+    if (!p || !host_out) return fail(MDSM_ERR_INVALID, "NULL argument");
+    size_t o = 0;
+    for (const auto &rh : p->ranges) {
+        std::memcpy(host_out + o, rh.dm_values.data(), rh.dm_values.size() * sizeof(float));
+        o += rh.dm_values.size();
+    }
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_dedisperse(mdsm_plan *p, const void *d_in, int64_t in_stride, float *d_out,
+                               void *d_work, void *stream)
+{
+    if (!p || !d_in || !d_out) return fail(MDSM_ERR_INVALID, "NULL argument");
+    if (in_stride < (int64_t)p->obs.nsamp_in)
+        return fail(MDSM_ERR_INVALID, "in_stride < nsamp_in");
+    int cur = 0;
+    hipError_t e;
+    if (!p->uploaded) {
+        int rc = upload(p);
+        if (rc) return rc;
+    }
+    e = hipGetDevice(&cur);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (cur != p->device) return fail(MDSM_ERR_STATE, "plan belongs to another device");
+    hipStream_t s = (hipStream_t)stream;
+    void *work = d_work;
+    if (!work && p->work_bytes) {
+        if (!p->own_work) {
+            e = hipMalloc(&p->own_work, p->work_bytes);
+            if (e != hipSuccess) {
+                p->own_work = nullptr;
+                return fail(MDSM_ERR_NOMEM, "workspace allocation of " +
+                                                std::to_string(p->work_bytes) + " bytes failed");
+            }
+        }
+        work = p->own_work;
+    }
+    for (const DecimStep &d : p->decim) {
+        TimedLaunch tl("decimate", s);
+        e = launch_decimate(d_in, d.dtype_in, p->obs.nchans, in_stride, p->obs.nsamp_in,
+                            (uint32_t)d.k, (char *)work + d.work_off, d.dtype_out, d.stride, s);
+        if (e != hipSuccess) return hip_fail(e, "decimate launch");
+    }
+    for (const Stage &st : p->stages) {
+        TimedLaunch tl("dedisperse", s);
+        e = launch_dedisperse(p, st, d_in, in_stride, d_out, work, s);
+        if (e != hipSuccess) return hip_fail(e, "dedisperse launch");
+    }
+    return MDSM_OK;
+}

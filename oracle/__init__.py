@@ -1,0 +1,288 @@
+"""ctypes bindings for the oracle (CPU restatement of MDSM's dedispersion path).
+
+TEST INFRASTRUCTURE ONLY: the parity checker.  Only tests/, the smoke()
+entry point and bench.py's cpu_baseline leg import this package; the
+product path (mdsm_amd) never does.
+
+``lib()`` loads oracle/liboracle.so (built by ``make -C oracle``).
+``ref()`` loads oracle/_ref/libmdsm_ref.so, the reference's own
+cpu_dedisp.c / dedispersion_output.cpp / file_handler.cpp compiled by
+``make -C oracle ref``; it returns None when that build is absent.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+U8, F16, F32 = 0, 1, 2
+_DT = {np.dtype(np.uint8): U8, np.dtype(np.float16): F16, np.dtype(np.float32): F32}
+
+_lib = None
+_ref = None
+
+
+class Cand(C.Structure):
+    _fields_ = [("dm", C.c_uint32), ("t", C.c_uint32), ("width", C.c_uint32),
+                ("snr", C.c_float)]
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise RuntimeError("oracle/liboracle.so missing: run make -C oracle")
+        L = C.CDLL(path)
+        vp, i32, i64, f32, f64 = C.c_void_p, C.c_int, C.c_int64, C.c_float, C.c_double
+        L.orc_dmdelay.restype = f32
+        L.orc_dmdelay.argtypes = [f32, f32]
+        L.orc_delay_table.argtypes = [f32, f32, i32, vp]
+        L.orc_shift_table_ref.argtypes = [vp, i32, f32, f32, i32, f32, vp]
+        L.orc_shift_table_ns.argtypes = [f64, f64, i32, f64, f64, i32, f64, vp]
+        L.orc_half_to_float.restype = f32
+        L.orc_half_to_float.argtypes = [C.c_uint16]
+        L.orc_dedisperse_ref_tm.argtypes = [vp, i32, i32, i32, vp, vp]
+        L.orc_dedisperse_f32.argtypes = [vp, i32, i32, i64, i32, i32, vp, vp, i64]
+        L.orc_dedisperse_f64.argtypes = [vp, i32, i32, i64, i32, i32, vp, vp, i64]
+        L.orc_decimate_f64.argtypes = [vp, i32, i32, i64, i32, i32, vp]
+        L.orc_range_f32.argtypes = [vp, i32, i32, i64, i32, i32, i32, i32, vp, vp, i64]
+        L.orc_range_f64.argtypes = [vp, i32, i32, i64, i32, i32, i32, i32, vp, vp, i64]
+        L.orc_subband_tables.argtypes = [vp, i32, i32, f32, f32, f32, i32, i32, f32, vp, vp]
+        L.orc_subband_f64.argtypes = [vp, i32, i32, i64, i32, i32, i32, i32, i32, vp, vp,
+                                      i32, i32, vp, i64]
+        L.orc_clip_u8.argtypes = [vp, i32, i64, i64, i32, f32, vp, i64, vp, vp, vp]
+        L.orc_row_stats.argtypes = [vp, i32, C.POINTER(f64), C.POINTER(f64)]
+        L.orc_boxcar_detect.restype = i32
+        L.orc_boxcar_detect.argtypes = [vp, i32, i32, i64, vp, i32, f64, vp, i32, vp, vp]
+        L.orc_median_of_five.restype = f32
+        L.orc_median_of_five.argtypes = [f32] * 5
+        _lib = L
+    return _lib
+
+
+# ---------------------------------------------------------------- shifts
+def delay_table(fch1, foff, nchans):
+    d = np.empty(nchans, np.float32)
+    lib().orc_delay_table(fch1, foff, nchans, _p(d))
+    return d
+
+
+def shift_table_ref(fch1, foff, nchans, startdm, dmstep, ndms, tsamp):
+    d = delay_table(fch1, foff, nchans)
+    s = np.empty((ndms, nchans), np.int32)
+    lib().orc_shift_table_ref(_p(d), nchans, startdm, dmstep, ndms, tsamp, _p(s))
+    return s
+
+
+def shift_table_ns(fch1, foff, nchans, startdm, dmstep, ndms, tsamp):
+    s = np.empty((ndms, nchans), np.int32)
+    lib().orc_shift_table_ns(fch1, foff, nchans, startdm, dmstep, ndms, tsamp, _p(s))
+    return s
+
+
+# ---------------------------------------------------------------- brute force
+def _cm(x):
+    x = np.ascontiguousarray(x)
+    return x, _DT[x.dtype], x.shape[0], x.shape[1]
+
+
+def dedisperse_ref_tm(x_tm, shift, nsamp):
+    """cpu_dedisp.c loop on time-major input [S][C]; returns time-major [nsamp][D]."""
+    x_tm = np.ascontiguousarray(x_tm, np.float32)
+    shift = np.ascontiguousarray(shift, np.int32)
+    ndms, nchans = shift.shape
+    out = np.empty((nsamp, ndms), np.float32)
+    lib().orc_dedisperse_ref_tm(_p(x_tm), nchans, nsamp, ndms, _p(shift), _p(out))
+    return out
+
+
+def dedisperse_f32(x, shift, nout):
+    x, dt, nchans, stride = _cm(x)
+    shift = np.ascontiguousarray(shift, np.int32)
+    ndms = shift.shape[0]
+    out = np.empty((ndms, nout), np.float32)
+    lib().orc_dedisperse_f32(_p(x), dt, nchans, stride, nout, ndms, _p(shift), _p(out), nout)
+    return out
+
+
+def dedisperse_f64(x, shift, nout):
+    x, dt, nchans, stride = _cm(x)
+    shift = np.ascontiguousarray(shift, np.int32)
+    ndms = shift.shape[0]
+    out = np.empty((ndms, nout), np.float64)
+    lib().orc_dedisperse_f64(_p(x), dt, nchans, stride, nout, ndms, _p(shift), _p(out), nout)
+    return out
+
+
+def decimate_f64(x, k):
+    x, dt, nchans, stride = _cm(x)
+    out = np.empty((nchans, stride // k), np.float64)
+    lib().orc_decimate_f64(_p(x), dt, nchans, stride, stride, k, _p(out))
+    return out
+
+
+def range_f32(x, k, shift, nout):
+    x, dt, nchans, stride = _cm(x)
+    shift = np.ascontiguousarray(shift, np.int32)
+    ndms = shift.shape[0]
+    out = np.empty((ndms, nout), np.float32)
+    lib().orc_range_f32(_p(x), dt, nchans, stride, stride, k, nout, ndms, _p(shift), _p(out), nout)
+    return out
+
+
+def range_f64(x, k, shift, nout):
+    x, dt, nchans, stride = _cm(x)
+    shift = np.ascontiguousarray(shift, np.int32)
+    ndms = shift.shape[0]
+    out = np.empty((ndms, nout), np.float64)
+    lib().orc_range_f64(_p(x), dt, nchans, stride, stride, k, nout, ndms, _p(shift), _p(out), nout)
+    return out
+
+
+# ---------------------------------------------------------------- subband
+def subband_tables(fch1, foff, nchans, nsubs, lowdm, sub_dmstep, dmstep, ncalls,
+                   calldms, tsamp_k):
+    d = delay_table(fch1, foff, nchans)
+    coarse = np.empty((ncalls, nchans), np.int32)
+    fine = np.empty((ncalls * calldms, nsubs), np.int32)
+    lib().orc_subband_tables(_p(d), nchans, nsubs, lowdm, sub_dmstep, dmstep, ncalls,
+                             calldms, tsamp_k, _p(coarse), _p(fine))
+    return coarse, fine
+
+
+def subband_f64(x, k, nsubs, ncalls, calldms, coarse, fine, len1, nout):
+    x, dt, nchans, stride = _cm(x)
+    coarse = np.ascontiguousarray(coarse, np.int32)
+    fine = np.ascontiguousarray(fine, np.int32)
+    out = np.empty((ncalls * calldms, nout), np.float64)
+    lib().orc_subband_f64(_p(x), dt, nchans, stride, stride, k, nsubs, ncalls, calldms,
+                          _p(coarse), _p(fine), len1, nout, _p(out), nout)
+    return out
+
+
+# ---------------------------------------------------------------- clipper
+def clip_u8(x, t0, n, nsigma):
+    x = np.ascontiguousarray(x, np.uint8)
+    nchans, stride = x.shape
+    out = np.empty((nchans, n), np.uint8)
+    med = np.empty(nchans, np.int32)
+    mad = np.empty(nchans, np.int32)
+    ncl = np.empty(nchans, np.int32)
+    lib().orc_clip_u8(_p(x), nchans, stride, t0, n, nsigma, _p(out), n, _p(med), _p(mad),
+                      _p(ncl))
+    return out, med, mad, ncl
+
+
+# ---------------------------------------------------------------- detector
+def row_stats(row):
+    row = np.ascontiguousarray(row, np.float32)
+    m, s = C.c_double(), C.c_double()
+    lib().orc_row_stats(_p(row), row.size, C.byref(m), C.byref(s))
+    return m.value, s.value
+
+
+def boxcar_detect(x, widths, thr, cap=1 << 20):
+    x = np.ascontiguousarray(x, np.float32)
+    ndms, n = x.shape
+    w = np.ascontiguousarray(widths, np.int32)
+    while True:
+        cands = (Cand * cap)()
+        means = np.empty(ndms, np.float64)
+        sig = np.empty(ndms, np.float64)
+        cnt = lib().orc_boxcar_detect(_p(x), ndms, n, n, _p(w), w.size, thr,
+                                      C.cast(cands, C.c_void_p), cap, _p(means), _p(sig))
+        if cnt <= cap:
+            break
+        cap = cnt
+    arr = np.zeros(cnt, dtype=[("dm", np.uint32), ("t", np.uint32), ("width", np.uint32),
+                               ("snr", np.float32)])
+    if cnt:
+        raw = np.ctypeslib.as_array(cands)[:cnt]
+        for f in ("dm", "t", "width", "snr"):
+            arr[f] = raw[f]
+    return arr, means, sig
+
+
+def median_of_five(a, b, c, d, e):
+    return lib().orc_median_of_five(a, b, c, d, e)
+
+
+# ---------------------------------------------------------------- reference build
+def ref():
+    """The reference's own code (oracle/_ref/libmdsm_ref.so) or None."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(HERE, "_ref", "libmdsm_ref.so")
+        if not os.path.exists(path):
+            return None
+        R = C.CDLL(path)
+        R.dmdelay.restype = C.c_float
+        R.dmdelay.argtypes = [C.c_float, C.c_float]
+        R.mdsm_ref_cpu_dedisp_main.restype = C.c_int
+        R.mdsm_ref_cpu_dedisp_main.argtypes = [C.c_int, C.POINTER(C.c_char_p)]
+        R.mdsm_ref_hooks_reset.restype = None
+        R.mdsm_ref_hooks_set_input.argtypes = [C.c_void_p, C.c_size_t]
+        R.mdsm_ref_hooks_count.restype = C.c_int
+        R.mdsm_ref_hooks_ptr.restype = C.c_void_p
+        R.mdsm_ref_hooks_ptr.argtypes = [C.c_int]
+        R.mdsm_ref_hooks_size.restype = C.c_size_t
+        R.mdsm_ref_hooks_size.argtypes = [C.c_int]
+        med5 = getattr(R, "_Z12medianOfFivefffff")
+        med5.restype = C.c_float
+        med5.argtypes = [C.c_float] * 5
+        R.median_of_five = med5
+        _ref = R
+    return _ref
+
+
+def ref_cpu_dedisp(fch1, foff, tsamp, nchans, nsamp, startdm, dmstep, tdms,
+                   data_tm=None):
+    """Run the reference's cpu_dedisp.c main() in-process.
+
+    The reference hard-codes fch1/foff/tsamp as globals and then rescales
+    foff /= nchans, tsamp *= nchans (src/prototypes/cpu_dedisp.c:9,36-37);
+    the globals are pre-set so that the rescaled values equal the requested
+    ones exactly (the factor nchans must be a power of two).  When data_tm
+    (time-major float32 [nsamp+maxshift][nchans]) is given it replaces the
+    reference's constant 0.1 input.  Returns (shift[tdms][nchans],
+    output_tm[nsamp][tdms], input_tm).
+    """
+    R = ref()
+    if R is None:
+        raise RuntimeError("oracle/_ref not built")
+    assert nchans & (nchans - 1) == 0, "nchans must be a power of two"
+    g = lambda name, ct: ct.in_dll(R, name)
+    g("fch1", C.c_float).value = fch1
+    g("foff", C.c_float).value = np.float32(foff) * np.float32(nchans)
+    g("tsamp", C.c_float).value = np.float32(tsamp) / np.float32(nchans)
+    R.mdsm_ref_hooks_reset()
+    if data_tm is not None:
+        data_tm = np.ascontiguousarray(data_tm, np.float32)
+        R.mdsm_ref_hooks_set_input(_p(data_tm), data_tm.nbytes)
+    args = ["cpu", "-nchans", str(nchans), "-nsamp", str(nsamp), "-startdm", repr(float(startdm)),
+            "-dmstep", repr(float(dmstep)), "-tdms", str(tdms)]
+    argv = (C.c_char_p * len(args))(*[a.encode() for a in args])
+    R.mdsm_ref_cpu_dedisp_main(len(args), argv)
+    assert np.float32(g("foff", C.c_float).value) == np.float32(foff)
+    assert np.float32(g("tsamp", C.c_float).value) == np.float32(tsamp)
+    n = R.mdsm_ref_hooks_count()
+    assert n == tdms + 3, n
+    shift = np.empty((tdms, nchans), np.int32)
+    for i in range(tdms):
+        p = R.mdsm_ref_hooks_ptr(1 + i)
+        shift[i] = np.ctypeslib.as_array(C.cast(p, C.POINTER(C.c_int32)), (nchans,))
+    maxshift = int(shift[-1, -1])
+    pin = R.mdsm_ref_hooks_ptr(tdms + 1)
+    pout = R.mdsm_ref_hooks_ptr(tdms + 2)
+    inp = np.ctypeslib.as_array(C.cast(pin, C.POINTER(C.c_float)),
+                                ((nsamp + maxshift), nchans)).copy()
+    out = np.ctypeslib.as_array(C.cast(pout, C.POINTER(C.c_float)), (nsamp, tdms)).copy()
+    R.mdsm_ref_hooks_reset()
+    return shift, out, inp

@@ -1,0 +1,118 @@
+/*
+ * oracle.h -- CPU restatement of MDSM's incoherent-dedispersion path.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the parity checker for the
+ * gfx950 implementation in mdsm_amd/.  Only tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg may load it; the product path never does.
+ *
+ * Every function cites the reference file:line whose algorithm it restates
+ * (paths relative to the reference repository root).
+ *
+ * Pinning (how each function is checked independently of this restatement):
+ *   orc_dmdelay / orc_delay_table / orc_shift_table_ref
+ *       bit-exact against the reference's own cpu_dedisp.c main() built
+ *       under oracle/_ref (its shift rows are captured through a recording
+ *       allocator), against committed golden vectors, and against the
+ *       disagreement counts of independent FP64 rules.
+ *   orc_shift_table_ns
+ *       closed form in FP64; impulse placement checked end to end.
+ *   orc_dedisperse_ref_tm / orc_dedisperse_f32
+ *       bit-exact against the reference's own cpu_dedisp.c loop (oracle/_ref)
+ *       on seeded data; constant-per-channel known answer
+ *       (src/prototypes/brute_dedisp_imp_tex.cu:112-114,167-175).
+ *   orc_dedisperse_f64
+ *       impulse response, DM-0 channel sum, linearity, integer exactness.
+ *   orc_decimate_f64 / orc_range_f32
+ *       closed form (k-sample sums), reduces to brute force at k = 1.
+ *   orc_subband_tables / orc_subband_f64
+ *       reduces to brute force with one channel per subband; integer
+ *       exactness; hand-computed tiny case.
+ *   orc_clip_u8
+ *       against numpy sort-based median/MAD on tiny cases (ties, even and
+ *       odd lengths, constant channels).
+ *   orc_boxcar_detect
+ *       closed forms (spike: snr_w = h/sqrt(w) pattern; top hat), brute
+ *       force numpy on tiny inputs.
+ *   orc_median_of_five
+ *       all 5^5 tuples against the reference's own medianOfFive built under
+ *       oracle/_ref and against sort.
+ */
+#ifndef MDSM_ORACLE_H
+#define MDSM_ORACLE_H
+
+#include <stdint.h>
+#include <stddef.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+enum { ORC_U8 = 0, ORC_F16 = 1, ORC_F32 = 2 };
+
+/* --- shift tables --------------------------------------------------- */
+float orc_dmdelay(float f1, float f2);
+void orc_delay_table(float fch1, float foff, int nchans, float *delay);
+void orc_shift_table_ref(const float *delay, int nchans, float startdm,
+                         float dmstep, int ndms, float tsamp, int32_t *shift);
+void orc_shift_table_ns(double fch1, double foff, int nchans, double startdm,
+                        double dmstep, int ndms, double tsamp, int32_t *shift);
+
+/* --- element access -------------------------------------------------- */
+float orc_half_to_float(uint16_t h);
+double orc_load(const void *in, int dtype, size_t idx);
+
+/* --- brute force ----------------------------------------------------- */
+void orc_dedisperse_ref_tm(const float *in_tm, int nchans, int nsamp,
+                           int ndms, const int32_t *shift, float *out_tm);
+void orc_dedisperse_f32(const void *in, int dtype, int nchans,
+                        int64_t in_stride, int nout, int ndms,
+                        const int32_t *shift, float *out, int64_t out_stride);
+void orc_dedisperse_f64(const void *in, int dtype, int nchans,
+                        int64_t in_stride, int nout, int ndms,
+                        const int32_t *shift, double *out, int64_t out_stride);
+
+/* --- decimation and multi-range -------------------------------------- */
+void orc_decimate_f64(const void *in, int dtype, int nchans, int64_t in_stride,
+                      int nsamp_in, int k, double *out);
+void orc_range_f32(const void *in, int dtype, int nchans, int64_t in_stride,
+                   int nsamp_in, int k, int nout, int ndms,
+                   const int32_t *shift, float *out, int64_t out_stride);
+void orc_range_f64(const void *in, int dtype, int nchans, int64_t in_stride,
+                   int nsamp_in, int k, int nout, int ndms,
+                   const int32_t *shift, double *out, int64_t out_stride);
+
+/* --- two-stage subband ----------------------------------------------- */
+void orc_subband_tables(const float *delay, int nchans, int nsubs,
+                        float lowdm, float sub_dmstep, float dmstep,
+                        int ncalls, int calldms, float tsamp_k,
+                        int32_t *coarse, int32_t *fine);
+void orc_subband_f64(const void *in, int dtype, int nchans, int64_t in_stride,
+                     int nsamp_in, int k, int nsubs, int ncalls, int calldms,
+                     const int32_t *coarse, const int32_t *fine, int len1,
+                     int nout, double *out, int64_t out_stride);
+
+/* --- RFI clipper ----------------------------------------------------- */
+void orc_clip_u8(const uint8_t *in, int nchans, int64_t in_stride, int64_t t0,
+                 int n, float nsigma, uint8_t *out, int64_t out_stride,
+                 int32_t *med, int32_t *mad, int32_t *nclipped);
+
+/* --- boxcar single-pulse detector ------------------------------------ */
+typedef struct {
+    uint32_t dm;
+    uint32_t t;
+    uint32_t width;
+    float snr;
+} orc_cand;
+
+void orc_row_stats(const float *row, int n, double *mean, double *sigma);
+int orc_boxcar_detect(const float *x, int ndms, int n, int64_t stride,
+                      const int *widths, int nwidths, double thr,
+                      orc_cand *cands, int cap, double *means, double *sigmas);
+
+/* --- reference in-path detector (median of five) --------------------- */
+float orc_median_of_five(float a, float b, float c, float d, float e);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,144 @@
+"""GPU parity: dedispersion through the C ABI against the oracle.
+
+Bars: uint8 input -> bit-exact (integer sums below 2^24 are exact in float32
+in any order); float16/float32 input -> normwise relative error
+max|gpu - fp64| / max|fp64| <= 1e-5, with the
+max-abs error printed per case.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import rel_err
+
+pytestmark = pytest.mark.gpu
+TOL = 1e-5
+
+
+def _plan(M, C, S, dtype, ranges, fch1=1500.0, foff=-0.390625, tsamp=64e-6, rule=0):
+    return M.Plan.brute(C, fch1, foff, tsamp, S, dtype, ranges, shift_rule=rule)
+
+
+def _check_range(plan, r, x, got, dtype):
+    R = plan.ranges[r]
+    sh = plan.shift_table(r)
+    ref = oracle.range_f64(x, R.downsample, sh, R.nout)
+    if dtype == "u8":
+        exact = oracle.range_f32(x, R.downsample, sh, R.nout)
+        assert np.array_equal(got, exact), f"range {r}: uint8 path not bit-exact"
+        assert np.array_equal(got.astype(np.float64), ref.astype(np.float32).astype(np.float64))
+    else:
+        e, a = rel_err(got, ref)
+        print(f"range {r} dtype {dtype}: rel {e:.3e} maxabs {a:.3e}")
+        assert e <= TOL
+
+
+@pytest.mark.parametrize("dtype", ["u8", "f16", "f32"])
+def test_single_range_small(gpu, dtype):
+    M = gpu
+    C, S = 64, 3000
+    print(f"case single_range_small dtype={dtype} C={C} S={S}", flush=True)
+    if dtype == "u8":
+        x = M.synth_filterbank(C, S, "u8", seed=11, mean=128, sigma=16)
+    else:
+        x = M.synth_filterbank(C, S, dtype, seed=12, mean=0.0, sigma=1.0)
+    plan = _plan(M, C, S, dtype, [(0.0, 1.5, 70, 1)])
+    outs = plan.dedisperse_host(x)
+    _check_range(plan, 0, x, outs[0], dtype)
+
+
+def test_const_per_channel_kat(gpu):
+    """x[c][t] = c gives every output C(C-1)/2 (src/prototypes/brute_dedisp_imp_tex.cu:112-114,167-175)."""
+    M = gpu
+    C, S = 128, 5000
+    print("case const_kat", flush=True)
+    x = np.repeat(np.arange(C, dtype=np.float32)[:, None], S, axis=1)
+    plan = _plan(M, C, S, "f32", [(0.0, 2.0, 40, 1)])
+    out = plan.dedisperse_host(x)[0]
+    assert np.all(out == C * (C - 1) / 2)
+
+
+def test_impulse_reads_every_shift(gpu):
+    """x = delta(c0, t0): out[d][t] = 1 iff t = t0 - shift[d][c0] (end-to-end table check)."""
+    M = gpu
+    C, S = 96, 4096
+    print("case impulse", flush=True)
+    plan = _plan(M, C, S, "f32", [(0.0, 3.0, 64, 1)])
+    sh = plan.shift_table(0)
+    N = plan.ranges[0].nout
+    for c0 in (0, 37, C - 1):
+        x = np.zeros((C, S), np.float32)
+        t0 = 2000
+        x[c0, t0] = 1.0
+        out = plan.dedisperse_host(x)[0]
+        exp = np.zeros_like(out)
+        for d in range(sh.shape[0]):
+            t = t0 - sh[d, c0]
+            if 0 <= t < N:
+                exp[d, t] = 1.0
+        assert np.array_equal(out, exp), f"impulse at channel {c0}"
+
+
+def test_multi_range_decimation_u8(gpu):
+    M = gpu
+    C, S = 64, 8192
+    print("case multi_range_decimation_u8", flush=True)
+    x = M.synth_filterbank(C, S, "u8", seed=21, mean=128, sigma=16,
+                           pulses=[(120.0, 3000, 8, 4.0)])
+    plan = _plan(M, C, S, "u8", [(0.0, 1.0, 40, 1), (40.0, 2.0, 30, 2), (100.0, 4.0, 20, 4),
+                                 (180.0, 5.0, 9, 3)])
+    outs = plan.dedisperse_host(x)
+    for r in range(4):
+        _check_range(plan, r, x, outs[r], "u8")
+
+
+@pytest.mark.parametrize("dtype", ["f16", "f32"])
+def test_multi_range_decimation_float(gpu, dtype):
+    M = gpu
+    C, S = 48, 6000
+    print(f"case multi_range_decimation_float {dtype}", flush=True)
+    x = M.synth_filterbank(C, S, dtype, seed=22, gain=(0.5, 2.0))
+    plan = _plan(M, C, S, dtype, [(0.0, 2.0, 33, 1), (66.0, 4.0, 17, 2), (130.0, 6.0, 11, 8)])
+    outs = plan.dedisperse_host(x)
+    for r in range(3):
+        _check_range(plan, r, x, outs[r], dtype)
+
+
+def test_ragged_stride_and_ns_rule(gpu):
+    """Row stride > nsamp, odd sizes, the NS shift rule."""
+    M = gpu
+    C, S, stride = 33, 3001, 3333
+    print("case ragged_stride_ns", flush=True)
+    x = M.synth_filterbank(C, S, "u8", seed=5, mean=100, sigma=20, stride=stride)
+    plan = _plan(M, C, S, "u8", [(3.0, 1.7, 45, 1)], rule=1)
+    sh = plan.shift_table(0)
+    ns = oracle.shift_table_ns(1500.0, -0.390625, C, np.float32(3.0), np.float32(1.7), 45, np.float32(64e-6))
+    assert np.array_equal(sh, ns)
+    din = M.DeviceBuffer.from_host(x)
+    dout = M.DeviceBuffer(plan.info.out_elems * 4)
+    plan.dedisperse(din, stride, dout)
+    M.synchronize()
+    got = dout.download(np.float32, (plan.info.out_elems,)).reshape(45, -1)
+    exact = oracle.range_f32(np.ascontiguousarray(x[:, :S]), 1, sh, plan.ranges[0].nout)
+    assert np.array_equal(got, exact)
+
+
+def test_config1_full_size(gpu):
+    """Config 1 at full size (256 ch, 65536 samples, 512 DMs, f32, pulse at DM 56.8):
+    all DM rows compared to the FP64 oracle on a subset of 64 rows."""
+    M = gpu
+    C, S = 256, 65536
+    print("case config1_full", flush=True)
+    x = M.synth_filterbank(C, S, "f32", seed=1, pulses=[(56.8, 30000, 4, 0.5)])
+    plan = _plan(M, C, S, "f32", [(0.0, 0.2, 512, 1)])
+    assert plan.ranges[0].maxshift == 433 and plan.ranges[0].nout == 65103
+    out = plan.dedisperse_host(x)[0]
+    sh = plan.shift_table(0)
+    rows = np.arange(0, 512, 8)
+    ref = oracle.dedisperse_f64(x, sh[rows], plan.ranges[0].nout)
+    e, a = rel_err(out[rows], ref)
+    print(f"config1 rel {e:.3e} maxabs {a:.3e}")
+    assert e <= TOL
+    # the injected pulse is the brightest point near DM 56.8 (row 284)
+    d, t = np.unravel_index(np.argmax(out), out.shape)
+    assert abs(d - 284) <= 3 and abs(t - 30000) <= 4
