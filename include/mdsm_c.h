@@ -212,14 +212,20 @@ int mdsm_detect_boxcar(const float *d_x, uint32_t ndms, uint32_t n,
                        void *stream);
 
 /* Median-of-five threshold detector of the reference's output thread
- * (src/mdsm/dedispersion_output.cpp:14-53,135-213): per row d, for
- * l < n-4, m = median(x[l..l+4]); emit (d, l, 5, m/sigma) when
- * m - mean >= thr*sigma and dm_values[d] > 1.0.  mean/sigma are the
- * FP64 statistics over the whole block (all rows), narrowed to float as the
- * reference's localmean/localrms.  d_stats: device scratch of 2+2*ndms
- * doubles; d_stats[0..1] receive mean and sigma. */
+ * (src/mdsm/dedispersion_output.cpp:14-53,56-213): per row d, for l < n-4,
+ * m = median(x[l..l+4]); a candidate (d, l, 5, snr) where dm_values[d] > 1.0
+ * and m - mean >= rms*thr (float arithmetic, as the reference).  Statistics:
+ *   chunk arrays given  : mean = d_chunk_mean[l/spc], rms = d_chunk_rms[l/spc],
+ *                         snr = (m-mean)/rms   (process_brute with noise arrays);
+ *   per_row_stats != 0  : per-row mean/rms of the n samples (process_subband);
+ *   otherwise           : mean/rms over the whole block, FP64 narrowed to float
+ *                         (process_brute without noise arrays);  snr = m/rms.
+ * d_stats: device scratch of 2+2*ndms doubles (d_stats[0..1] = block mean and
+ * sigma in the whole-block mode).  *d_count is zeroed by the call. */
 int mdsm_detect_median5(const float *d_x, uint32_t ndms, uint32_t n,
                         int64_t stride, const float *d_dm_values, double thr,
+                        int32_t per_row_stats, const float *d_chunk_mean,
+                        const float *d_chunk_rms, uint32_t samples_per_chunk,
                         mdsm_cand *d_cands, uint32_t cap, uint32_t *d_count,
                         double *d_stats, void *stream);
 

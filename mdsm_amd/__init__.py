@@ -125,7 +125,8 @@ def lib():
         "mdsm_decimate": [vp, i32, u32, i64, u32, u32, vp, i64, vp],
         "mdsm_clip_u8": [vp, i64, u32, i64, u32, f32, vp, i64, vp, vp, vp, vp],
         "mdsm_detect_boxcar": [vp, u32, u32, i64, vp, u32, f64, vp, u32, vp, vp, vp, vp],
-        "mdsm_detect_median5": [vp, u32, u32, i64, vp, f64, vp, u32, vp, vp, vp],
+        "mdsm_detect_median5": [vp, u32, u32, i64, vp, f64, i32, vp, vp, u32, vp, u32, vp, vp,
+                                vp],
         "mdsm_synth_filterbank": [C.POINTER(SynthSpec), vp, i64],
         "mdsm_timing_enable": [C.c_int],
         "mdsm_timing_reset": [],
@@ -408,10 +409,12 @@ def detect_boxcar(d_x, ndms, n, stride, widths, thr, d_cands, cap, d_count, d_me
 
 
 def detect_median5(d_x, ndms, n, stride, d_dmv, thr, d_cands, cap, d_count, d_stats,
+                   per_row=False, d_chunk_mean=None, d_chunk_rms=None, samples_per_chunk=0,
                    stream=None):
     g = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
-    check(lib().mdsm_detect_median5(g(d_x), ndms, n, stride, g(d_dmv), thr, g(d_cands), cap,
-                                    g(d_count), g(d_stats), _sh(stream)))
+    check(lib().mdsm_detect_median5(g(d_x), ndms, n, stride, g(d_dmv), thr, 1 if per_row else 0,
+                                    g(d_chunk_mean), g(d_chunk_rms), samples_per_chunk,
+                                    g(d_cands), cap, g(d_count), g(d_stats), _sh(stream)))
 
 
 def read_candidates(d_cands, d_count, cap) -> np.ndarray:

@@ -160,6 +160,7 @@ extern "C" int mdsm_detect_boxcar(const float *d_x, uint32_t ndms, uint32_t n, i
     if (!d_x || !widths || !d_cands || !d_count || !d_mean || !d_sigma)
         return fail(MDSM_ERR_INVALID, "NULL argument");
     if (stride < (int64_t)n || n < 1) return fail(MDSM_ERR_INVALID, "bad row length/stride");
+    if (ndms > 65535) return fail(MDSM_ERR_INVALID, "ndms > 65535");
     uint32_t mask = 0;
     for (uint32_t i = 0; i < nwidths; ++i) {
         uint32_t w = widths[i];
@@ -178,15 +179,25 @@ extern "C" int mdsm_detect_boxcar(const float *d_x, uint32_t ndms, uint32_t n, i
 }
 
 extern "C" int mdsm_detect_median5(const float *d_x, uint32_t ndms, uint32_t n, int64_t stride,
-                                   const float *d_dm_values, double thr, mdsm_cand *d_cands,
-                                   uint32_t cap, uint32_t *d_count, double *d_stats, void *stream)
+                                   const float *d_dm_values, double thr, int32_t per_row_stats,
+                                   const float *d_chunk_mean, const float *d_chunk_rms,
+                                   uint32_t samples_per_chunk, mdsm_cand *d_cands, uint32_t cap,
+                                   uint32_t *d_count, double *d_stats, void *stream)
 {
     if (!d_x || !d_dm_values || !d_cands || !d_count || !d_stats)
         return fail(MDSM_ERR_INVALID, "NULL argument");
     if (stride < (int64_t)n) return fail(MDSM_ERR_INVALID, "bad row length/stride");
+    if (ndms > 65535) return fail(MDSM_ERR_INVALID, "ndms > 65535");
+    int mode = per_row_stats ? 1 : 0;
+    if (d_chunk_mean || d_chunk_rms) {
+        if (!d_chunk_mean || !d_chunk_rms || samples_per_chunk == 0)
+            return fail(MDSM_ERR_INVALID, "chunk statistics need mean, rms and samples_per_chunk");
+        mode = 2;
+    }
     TimedLaunch tl("median5", (hipStream_t)stream);
-    hipError_t e = launch_median5(d_x, ndms, n, stride, d_dm_values, thr, d_cands, cap, d_count,
-                                  d_stats, (hipStream_t)stream);
+    hipError_t e = launch_median5(d_x, ndms, n, stride, d_dm_values, thr, mode, d_chunk_mean,
+                                  d_chunk_rms, samples_per_chunk, d_cands, cap, d_count, d_stats,
+                                  (hipStream_t)stream);
     if (e != hipSuccess) return hip_fail(e, "median5 launch");
     return MDSM_OK;
 }

@@ -10,22 +10,52 @@
 //   contiguous run: blocks sharing an input tile meet in the same L2.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "mdsm_internal.h"
 
 namespace mdsm {
 
+// Small fill kernel: used instead of hipMemsetAsync so that the library's
+// launch sequences can be captured into HIP graphs.
+__global__ void fill_u32_kernel(uint32_t *p, uint32_t n, uint32_t v)
+{
+    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s)
+{
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((n + 255) / 256 > 1024 ? 1024 : (n + 255) / 256),
+                       dim3(256), 0, s, p, n, v);
+    return hipGetLastError();
+}
+
+// Loads through the constant address space are scalar (s_load) when the
+// address is wave-uniform: the per-(row, DM) offsets are read this way.
+typedef __attribute__((address_space(4))) const int32_t cint32_t;
+__device__ __forceinline__ const cint32_t *as_const(const int32_t *p)
+{
+    return (const cint32_t *)(p);
+}
+
 struct DedispArgs {
-    const BlockDesc *blocks;
     const SegDesc *segs;
+    int64_t items;
+    int32_t seg_begin;
+    int32_t nseg;
+    uint32_t *counter;  // persistent mode when non-NULL
+    JobBufs jobs[kMaxJobs];
+};
+
+// What a work item resolves to: its block, the tables and buffers it uses.
+struct ItemCtx {
     const int32_t *rel;
     const int32_t *abase;
     const void *in;
     float *out;
     char *work;
     int64_t in_stride;
-    int64_t items;
-    int32_t seg_begin;
-    int32_t nseg;
 };
 
 template <int DT>
@@ -60,34 +90,167 @@ __device__ __forceinline__ void load_tile(float *tile, const void *src, int64_t 
     }
 }
 
-__global__ void __launch_bounds__(256) dedisperse_kernel(DedispArgs a)
+// ---- integer path (uint8 / uint16 blocks) --------------------------------
+// Row i of a chunk holds 4 phases x K pairs (K = W/4) of 8 bytes; pair k of
+// phase s packs samples u = 4k+s+{0,1,2,3} as uint16 lanes:
+//   .x = x[u] | x[u+2] << 16,   .y = x[u+1] | x[u+3] << 16.
+// A DM offset o then reads one aligned pair: phase o&3, index (o>>2) + lane,
+// giving the lane's 4 consecutive samples; two channels accumulate with one
+// v_add3_u32 per half.  Chunks hold <= 16 rows of values <= 4095, so the
+// uint16 lanes cannot overflow before the per-chunk flush to float32.
+__device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
 {
-    extern __shared__ float tile[];
+    return __builtin_amdgcn_perm(hi, lo, sel);
+}
 
-    // XCD-aware remap: XCD x = blockIdx % 8 gets a contiguous run of items.
-    const int64_t g = blockIdx.x;
-    const int64_t q = a.items >> 3, rem = a.items & 7;
-    const int64_t xcd = g & 7, loc = g >> 3;
-    const int64_t f = xcd * q + (xcd < rem ? xcd : rem) + loc;
+template <int DT>
+__device__ __forceinline__ uint32_t load_u(const void *base, int64_t i)
+{
+    if constexpr (DT == DT_U8)
+        return ((const uint8_t *)base)[i];
+    else
+        return ((const uint16_t *)base)[i];
+}
 
-    // Segment lookup (wave-uniform binary search over item_start).
-    int lo = a.seg_begin, hi = a.seg_begin + a.nseg - 1;
-    while (lo < hi) {
-        const int mid = (lo + hi + 1) >> 1;
-        if (a.segs[mid].item_start <= f)
-            lo = mid;
-        else
-            hi = mid - 1;
+template <int DT>
+__device__ __forceinline__ void load_tile_int(uint2 *lds, const void *src, int64_t in_off,
+                                              int64_t row0, int64_t rs, const int32_t *ab, int t0,
+                                              int W, int cc, int in_len, int wave, int lane)
+{
+    const int K = W >> 2;
+    for (int i = wave; i < cc; i += kWaves) {
+        const int64_t rowbase = in_off + (row0 + i) * rs;
+        const int col0 = ab[i] + t0;
+        uint2 *dst = lds + (size_t)i * 4 * K;
+        const bool aligned = ((rowbase * (DT == DT_U8 ? 1 : 2)) & 3) == 0;
+        for (int k = lane; k < K; k += 64) {
+            const int col = col0 + 4 * k;
+            uint32_t d0[4], d1[4];
+            if (aligned && col + 8 <= in_len) {
+                if constexpr (DT == DT_U8) {
+                    const uint32_t *p = (const uint32_t *)((const uint8_t *)src + rowbase + col);
+                    const uint32_t a = p[0], b = p[1];  // samples 0-3, 4-7
+#pragma unroll
+                    for (int ph = 0; ph < 4; ++ph) {
+                        d0[ph] = perm(b, a, 0x0c000c00u | (uint32_t)ph | ((uint32_t)(ph + 2) << 16));
+                        d1[ph] = perm(b, a, 0x0c000c00u | (uint32_t)(ph + 1) | ((uint32_t)(ph + 3) << 16));
+                    }
+                } else {
+                    const uint32_t *p = (const uint32_t *)((const uint16_t *)src + rowbase + col);
+                    const uint32_t h0 = p[0], h1 = p[1], h2 = p[2], h3 = p[3];
+                    const uint32_t P0 = perm(h1, h0, 0x05040100u), Q0 = perm(h1, h0, 0x07060302u);
+                    const uint32_t P1 = perm(h2, h1, 0x05040100u), Q1 = perm(h2, h1, 0x07060302u);
+                    const uint32_t P2 = perm(h3, h2, 0x05040100u);
+                    d0[0] = P0; d1[0] = Q0;
+                    d0[1] = Q0; d1[1] = P1;
+                    d0[2] = P1; d1[2] = Q1;
+                    d0[3] = Q1; d1[3] = P2;
+                }
+            } else {
+                uint32_t v[8];
+#pragma unroll
+                for (int j = 0; j < 8; ++j)
+                    v[j] = (col + j < in_len) ? load_u<DT>(src, rowbase + col + j) : 0u;
+#pragma unroll
+                for (int ph = 0; ph < 4; ++ph) {
+                    d0[ph] = v[ph] | (v[ph + 2] << 16);
+                    d1[ph] = v[ph + 1] | (v[ph + 3] << 16);
+                }
+            }
+#pragma unroll
+            for (int ph = 0; ph < 4; ++ph) dst[ph * K + k] = make_uint2(d0[ph], d1[ph]);
+        }
     }
-    const SegDesc sg = a.segs[lo];
-    const int64_t fl = f - sg.item_start;
-    const int tile_i = (int)(fl / sg.nblocks);
-    const int bi = sg.first_block + (int)(fl - (int64_t)tile_i * sg.nblocks);
-    const BlockDesc bd = a.blocks[bi];
+}
 
-    const int lane = threadIdx.x & 63;
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int t0 = tile_i * kTile;
+__device__ void dedisperse_int(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
+                               int lane, uint2 *lds)
+{
+    const int W = bd.W, K = W >> 2;
+    const int64_t rs = bd.src == BUF_IN ? a.in_stride : bd.row_stride;
+    const void *src = bd.src == BUF_IN ? a.in : (const void *)a.work;
+    const int32_t *relw = a.rel + bd.tab_off + wave * kRPW;
+    const int32_t *ab = a.abase + bd.base_off;
+    float f[kRPW][4];
+#pragma unroll
+    for (int r = 0; r < kRPW; ++r)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) f[r][j] = 0.0f;
+
+    for (int c0 = 0; c0 < bd.nrows; c0 += bd.ccr) {
+        const int cc = min(bd.ccr, bd.nrows - c0);
+        __syncthreads();
+        if (bd.dtype == DT_U8)
+            load_tile_int<DT_U8>(lds, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, W, cc,
+                                 bd.in_len, wave, lane);
+        else
+            load_tile_int<DT_U16>(lds, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, W, cc,
+                                  bd.in_len, wave, lane);
+        __syncthreads();
+        uint32_t ae[kRPW], ao[kRPW];
+#pragma unroll
+        for (int r = 0; r < kRPW; ++r) ae[r] = ao[r] = 0u;
+        const int rb = 32 * K;  // bytes per staged row
+        const char *lane_base = (const char *)lds + lane * 8;
+        int i = 0;
+        for (; i + 1 < cc; i += 2) {
+            const cint32_t *ri = as_const(relw + (c0 + i) * kDB);
+            const cint32_t *rj = ri + kDB;
+            const char *rowi = lane_base + i * rb;
+            const char *rowj = rowi + rb;
+#pragma unroll
+            for (int r = 0; r < kRPW; ++r) {
+                const uint2 x = *(const uint2 *)(rowi + ri[r]);
+                const uint2 y = *(const uint2 *)(rowj + rj[r]);
+                ae[r] = ae[r] + x.x + y.x;
+                ao[r] = ao[r] + x.y + y.y;
+            }
+        }
+        if (i < cc) {
+            const cint32_t *ri = as_const(relw + (c0 + i) * kDB);
+            const char *rowi = lane_base + i * rb;
+#pragma unroll
+            for (int r = 0; r < kRPW; ++r) {
+                const uint2 x = *(const uint2 *)(rowi + ri[r]);
+                ae[r] += x.x;
+                ao[r] += x.y;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < kRPW; ++r) {
+            f[r][0] += (float)(ae[r] & 0xffffu);
+            f[r][2] += (float)(ae[r] >> 16);
+            f[r][1] += (float)(ao[r] & 0xffffu);
+            f[r][3] += (float)(ao[r] >> 16);
+        }
+    }
+
+    float *ob = (bd.dst == BUF_OUT ? a.out : (float *)a.work) + bd.out_off;
+    const float kf = (float)bd.k;
+    const int t = t0 + 4 * lane;
+#pragma unroll
+    for (int r = 0; r < kRPW; ++r) {
+        const int j = wave * kRPW + r;
+        if (j >= bd.ntrials) continue;
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = bd.k > 1 ? __fdiv_rn(f[r][q], kf) : f[r][q];
+        float *o = ob + (int64_t)j * bd.out_trial_stride;
+        const bool al = (((uintptr_t)(o + t0)) & 15) == 0;
+        if (al && t + 3 < bd.nout) {
+            *(float4 *)(o + t) = make_float4(v[0], v[1], v[2], v[3]);
+        } else {
+#pragma unroll
+            for (int q = 0; q < 4; ++q)
+                if (t + q < bd.nout) o[t + q] = v[q];
+        }
+    }
+}
+
+// ---- float path (float16 / float32 blocks) -------------------------------
+__device__ void dedisperse_float(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
+                                 int lane, float *tile)
+{
     const int W = bd.W;
     const int64_t rs = bd.src == BUF_IN ? a.in_stride : bd.row_stride;
     const void *src = bd.src == BUF_IN ? a.in : (const void *)a.work;
@@ -100,34 +263,22 @@ __global__ void __launch_bounds__(256) dedisperse_kernel(DedispArgs a)
 #pragma unroll
         for (int p = 0; p < kPPL; ++p) acc[r][p] = 0.0f;
 
-    for (int c0 = 0; c0 < bd.nrows; c0 += kCC) {
-        const int cc = min(kCC, bd.nrows - c0);
+    for (int c0 = 0; c0 < bd.nrows; c0 += bd.ccr) {
+        const int cc = min(bd.ccr, bd.nrows - c0);
         __syncthreads();
-        switch (bd.dtype) {
-        case DT_U8:
-            load_tile<DT_U8>(tile, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, W, cc, bd.in_len,
-                             wave, lane);
-            break;
-        case DT_U16:
-            load_tile<DT_U16>(tile, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, W, cc,
-                              bd.in_len, wave, lane);
-            break;
-        case DT_F16:
+        if (bd.dtype == DT_F16)
             load_tile<DT_F16>(tile, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, W, cc,
                               bd.in_len, wave, lane);
-            break;
-        default:
+        else
             load_tile<DT_F32>(tile, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, W, cc,
                               bd.in_len, wave, lane);
-            break;
-        }
         __syncthreads();
         for (int i = 0; i < cc; ++i) {
-            const float *row = tile + i * W + lane;
-            const int32_t *rl = relw + (c0 + i) * kDB;
+            const char *row = (const char *)(tile + i * W + lane);
+            const cint32_t *rl = as_const(relw + (c0 + i) * kDB);
 #pragma unroll
             for (int r = 0; r < kRPW; ++r) {
-                const float *qp = row + rl[r];
+                const float *qp = (const float *)(row + rl[r]);
 #pragma unroll
                 for (int p = 0; p < kPPL; ++p) acc[r][p] += qp[64 * p];
             }
@@ -150,26 +301,93 @@ __global__ void __launch_bounds__(256) dedisperse_kernel(DedispArgs a)
     }
 }
 
+__device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, int wave,
+                                                int lane, uint2 *lds)
+{
+    // Segment lookup (wave-uniform binary search over item_start).
+    int lo = a.seg_begin, hi = a.seg_begin + a.nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (a.segs[mid].item_start <= f)
+            lo = mid;
+        else
+            hi = mid - 1;
+    }
+    const SegDesc sg = a.segs[lo];
+    const int64_t fl = f - sg.item_start;
+    const int tile_i = (int)(fl / sg.nblocks);
+    const int bi = sg.first_block + (int)(fl - (int64_t)tile_i * sg.nblocks);
+    const BlockDesc bd = sg.blocks[bi];
+    const JobBufs jb = a.jobs[sg.job];
+    ItemCtx ctx{sg.rel, sg.abase, jb.in, jb.out, jb.work, jb.in_stride};
+    const int t0 = tile_i * kTile;
+    if (bd.dtype == DT_U8 || bd.dtype == DT_U16)
+        dedisperse_int(ctx, bd, t0, wave, lane, lds);
+    else
+        dedisperse_float(ctx, bd, t0, wave, lane, (float *)lds);
+}
+
+__global__ void __launch_bounds__(256) dedisperse_kernel(DedispArgs a)
+{
+    extern __shared__ uint2 lds_raw[];
+    __shared__ int64_t sh_item;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    if (a.counter == nullptr) {
+        // One workgroup per item; XCD-aware remap: XCD x = blockIdx % 8 gets a
+        // contiguous run of items, so blocks sharing an input tile share an L2.
+        const int64_t g = blockIdx.x;
+        const int64_t q = a.items >> 3, rem = a.items & 7;
+        const int64_t xcd = g & 7, loc = g >> 3;
+        const int64_t f = xcd * q + (xcd < rem ? xcd : rem) + loc;
+        dedisperse_item(a, f, wave, lane, lds_raw);
+        return;
+    }
+    // Persistent mode: workgroups pull items from a device counter until the
+    // queue is empty; every workgroup reaches the exit test.
+    for (;;) {
+        __syncthreads();
+        if (threadIdx.x == 0) sh_item = (int64_t)atomicAdd(a.counter, 1u);
+        __syncthreads();
+        const int64_t f = sh_item;
+        if (f >= a.items) break;
+        dedisperse_item(a, f, wave, lane, lds_raw);
+    }
+}
+
+hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s)
+{
+    if (L.items <= 0) return hipSuccess;
+    if (L.items > 0x7fffffffLL || L.njobs > kMaxJobs) return hipErrorInvalidValue;
+    DedispArgs a{};
+    a.segs = L.segs;
+    a.items = L.items;
+    a.seg_begin = L.seg_begin;
+    a.nseg = L.nseg;
+    a.counter = L.counter;
+    for (int i = 0; i < L.njobs; ++i) a.jobs[i] = L.jobs[i];
+    const size_t lds = (size_t)L.lds_bytes;
+    unsigned grid = (unsigned)L.items;
+    if (L.counter) {
+        hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, L.counter, 1u, 0u);
+        grid = (unsigned)std::min<int64_t>(L.grid > 0 ? L.grid : 2048, L.items);
+    }
+    hipLaunchKernelGGL(dedisperse_kernel, dim3(grid), dim3(256), lds, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in,
                              int64_t in_stride, float *out, void *work, hipStream_t s)
 {
-    if (st.items <= 0) return hipSuccess;
-    if (st.items > 0x7fffffffLL) return hipErrorInvalidValue;
-    DedispArgs a;
-    a.blocks = p->dev.blocks;
-    a.segs = p->dev.segs;
-    a.rel = p->dev.rel;
-    a.abase = p->dev.abase;
-    a.in = in;
-    a.out = out;
-    a.work = (char *)work;
-    a.in_stride = in_stride;
-    a.items = st.items;
-    a.seg_begin = st.seg_begin;
-    a.nseg = st.seg_end - st.seg_begin;
-    const size_t lds = (size_t)kCC * st.wmax * sizeof(float);
-    hipLaunchKernelGGL(dedisperse_kernel, dim3((unsigned)st.items), dim3(256), lds, s, a);
-    return hipGetLastError();
+    LaunchSpec L;
+    L.segs = p->dev.segs;
+    L.seg_begin = st.seg_begin;
+    L.nseg = st.seg_end - st.seg_begin;
+    L.items = st.items;
+    L.lds_bytes = st.lds_bytes;
+    L.njobs = 1;
+    L.jobs[0] = JobBufs{in, out, (char *)work, in_stride};
+    return launch_dedisperse_spec(L, s);
 }
 
 // ---------------------------------------------------------------------------
@@ -188,7 +406,10 @@ __global__ void __launch_bounds__(256)
     if constexpr (DTI == DT_U8) {
         uint32_t acc = 0;
         for (uint32_t j = 0; j < k; ++j) acc += ((const uint8_t *)in)[base + j];
-        ((uint16_t *)out)[c * out_stride + t] = (uint16_t)acc;
+        if constexpr (DTO == DT_U16)
+            ((uint16_t *)out)[c * out_stride + t] = (uint16_t)acc;
+        else
+            ((float *)out)[c * out_stride + t] = (float)acc;
     } else {
         float acc = 0.0f;
         for (uint32_t j = 0; j < k; ++j) acc += load_sample<DTI>(in, base + j);
@@ -206,6 +427,9 @@ hipError_t launch_decimate(const void *in, int dtype_in, uint32_t nchans, int64_
     if (dtype_in == DT_U8 && dtype_out == DT_U16)
         hipLaunchKernelGGL((decimate_kernel<DT_U8, DT_U16>), grid, dim3(256), 0, s, in, in_stride,
                            nout, k, out, out_stride);
+    else if (dtype_in == DT_U8 && dtype_out == DT_F32)
+        hipLaunchKernelGGL((decimate_kernel<DT_U8, DT_F32>), grid, dim3(256), 0, s, in, in_stride,
+                           nout, k, out, out_stride);
     else if (dtype_in == DT_F16 && dtype_out == DT_F32)
         hipLaunchKernelGGL((decimate_kernel<DT_F16, DT_F32>), grid, dim3(256), 0, s, in,
                            in_stride, nout, k, out, out_stride);
@@ -214,21 +438,6 @@ hipError_t launch_decimate(const void *in, int dtype_in, uint32_t nchans, int64_
                            in_stride, nout, k, out, out_stride);
     else
         return hipErrorInvalidValue;
-    return hipGetLastError();
-}
-
-// Small fill kernel: used instead of hipMemsetAsync so that the library's
-// launch sequences can be captured into HIP graphs.
-__global__ void fill_u32_kernel(uint32_t *p, uint32_t n, uint32_t v)
-{
-    for (uint32_t i = threadIdx.x; i < n; i += blockDim.x) p[i] = v;
-}
-
-hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s)
-{
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(fill_u32_kernel, dim3((n + 255) / 256 > 1024 ? 1024 : (n + 255) / 256),
-                       dim3(256), 0, s, p, n, v);
     return hipGetLastError();
 }
 
@@ -527,15 +736,33 @@ __global__ void block_stats_finish(const double *s1, const double *s2, uint32_t 
     }
 }
 
+// mode 0: statistics over the whole block (stats[0..1]);
+// mode 1: per-row statistics from the row sums (stats[2..2+2*ndms)), as the
+//         reference's subband output path (dedispersion_output.cpp:86-96);
+// mode 2: caller-supplied per-chunk mean/rms, chunk = l / spc
+//         (dedispersion_output.cpp:120-126,199-208).
 __global__ void __launch_bounds__(256)
     median5_kernel(const float *x, uint32_t n, int64_t stride, const float *dmv, float thr,
-                   mdsm_cand *cands, uint32_t cap, uint32_t *count, const double *stats)
+                   int mode, const float *cmean, const float *crms, uint32_t spc,
+                   mdsm_cand *cands, uint32_t cap, uint32_t *count, const double *stats,
+                   uint32_t ndms)
 {
     const uint32_t d = blockIdx.y;
     const uint32_t l = blockIdx.x * 256u + threadIdx.x;
     if (n < 5 || l >= n - 4) return;
     if (!(dmv[d] > 1.0f)) return;
-    const float lm = (float)stats[0], lr = (float)stats[1];
+    float lm, lr;
+    if (mode == 0) {
+        lm = (float)stats[0];
+        lr = (float)stats[1];
+    } else if (mode == 1) {
+        lm = (float)(stats[2 + d] / (double)n);
+        const double v = stats[2 + ndms + d] / (double)n - (double)lm * (double)lm;
+        lr = (float)sqrt(v);
+    } else {
+        lm = cmean[l / spc];
+        lr = crms[l / spc];
+    }
     const float *r = x + (int64_t)d * stride + l;
     const float m = med5(r[0], r[1], r[2], r[3], r[4]);
     if (m - lm >= lr * thr) {
@@ -545,25 +772,29 @@ __global__ void __launch_bounds__(256)
             cd.dm = d;
             cd.t = l;
             cd.width = 5;
-            cd.snr = m / lr;
+            cd.snr = mode == 2 ? (m - lm) / lr : m / lr;
             cands[slot] = cd;
         }
     }
 }
 
 hipError_t launch_median5(const float *x, uint32_t ndms, uint32_t n, int64_t stride,
-                          const float *dmv, double thr, mdsm_cand *cands, uint32_t cap,
+                          const float *dmv, double thr, int mode, const float *cmean,
+                          const float *crms, uint32_t spc, mdsm_cand *cands, uint32_t cap,
                           uint32_t *count, double *stats, hipStream_t s)
 {
-    if (ndms == 0 || n < 5) return hipSuccess;
-    // stats scratch: stats[2 .. 2+2*ndms) holds the per-row sums
-    double *s1 = stats + 2, *s2 = stats + 2 + ndms;
-    hipLaunchKernelGGL(row_sums_kernel, dim3(ndms), dim3(256), 0, s, x, n, stride, s1, s2);
-    hipLaunchKernelGGL(block_stats_finish, dim3(1), dim3(64), 0, s, s1, s2, ndms, n, stats);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, count, 1u, 0u);
+    if (ndms == 0 || n < 5) return hipGetLastError();
+    // stats[2 .. 2+2*ndms) holds the per-row sums
+    double *s1 = stats + 2, *s2 = stats + 2 + ndms;
+    if (mode != 2) {
+        hipLaunchKernelGGL(row_sums_kernel, dim3(ndms), dim3(256), 0, s, x, n, stride, s1, s2);
+        if (mode == 0)
+            hipLaunchKernelGGL(block_stats_finish, dim3(1), dim3(64), 0, s, s1, s2, ndms, n, stats);
+    }
     dim3 grid((n - 4 + 255) / 256, ndms);
-    hipLaunchKernelGGL(median5_kernel, grid, dim3(256), 0, s, x, n, stride, dmv, (float)thr, cands,
-                       cap, count, stats);
+    hipLaunchKernelGGL(median5_kernel, grid, dim3(256), 0, s, x, n, stride, dmv, (float)thr, mode,
+                       cmean, crms, spc, cands, cap, count, stats, ndms);
     return hipGetLastError();
 }
 

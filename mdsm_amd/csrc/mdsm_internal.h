@@ -53,17 +53,37 @@ struct BlockDesc {
     int32_t dtype;
     int32_t tab_off;          // int offset of rel[nrows][kDB] in the rel table
     int32_t base_off;         // int offset of abase[nrows] in the base table
-    int32_t pad;
+    int32_t ccr;              // rows staged per LDS chunk
 };
+
+// LDS bytes of one staged row: integer data (uint8/uint16) is held as four
+// phase-shifted copies of uint16 pairs (8 bytes per sample), float data as
+// float32 (4 bytes per sample).
+inline int lds_row_bytes(int dtype, int W) { return (dtype == DT_U8 || dtype == DT_U16) ? 8 * W : 4 * W; }
+constexpr int kLdsBudget = 65536;  // bytes of LDS per workgroup
 
 // A run of blocks tiled over the same number of output tiles; flat work
 // items of a segment are ordered tile-major (all blocks of tile 0 first).
+// Segments are self-describing so that one launch can mix plans: each names
+// its plan's device tables and the job (buffer set) it reads and writes.
 struct SegDesc {
     int64_t item_start;
+    const BlockDesc *blocks;  // device: the plan's block table
+    const int32_t *rel;       // device: the plan's offset table
+    const int32_t *abase;     // device: the plan's base table
     int32_t first_block;
     int32_t nblocks;
     int32_t ntiles;
-    int32_t pad;
+    int32_t job;              // index into the launch's job buffers
+};
+
+// Buffers of one job (one block of one beam) in a launch.
+constexpr int kMaxJobs = 32;
+struct JobBufs {
+    const void *in;
+    float *out;
+    char *work;
+    int64_t in_stride;
 };
 
 // One dedispersion launch: a contiguous run of segments.
@@ -71,6 +91,7 @@ struct Stage {
     int32_t seg_begin = 0, seg_end = 0;
     int64_t items = 0;
     int32_t wmax = 0;
+    int32_t lds_bytes = 0;   // dynamic LDS of the launch
 };
 
 struct DecimStep {
@@ -97,6 +118,18 @@ struct KernelTables {
     SegDesc *segs = nullptr;
     int32_t *rel = nullptr;
     int32_t *abase = nullptr;
+};
+
+// A dedispersion launch over segments already resident on the device.
+struct LaunchSpec {
+    const SegDesc *segs = nullptr;  // device
+    int32_t seg_begin = 0, nseg = 0;
+    int64_t items = 0;
+    int32_t lds_bytes = 0;
+    uint32_t *counter = nullptr;    // persistent mode: device work counter (zeroed here)
+    int32_t grid = 0;               // persistent mode: workgroups (0 = one per item)
+    int32_t njobs = 0;
+    JobBufs jobs[kMaxJobs];
 };
 
 }  // namespace mdsm
@@ -139,6 +172,7 @@ struct TimedLaunch {
 // Launchers (kernels.hip).
 hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in,
                              int64_t in_stride, float *out, void *work, hipStream_t s);
+hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s);
 hipError_t launch_decimate(const void *in, int dtype_in, uint32_t nchans, int64_t in_stride,
                            uint32_t nsamp, uint32_t k, void *out, int dtype_out,
                            int64_t out_stride, hipStream_t s);
@@ -154,7 +188,8 @@ hipError_t launch_boxcar(const float *x, uint32_t ndms, uint32_t n, int64_t stri
                          uint32_t cap, uint32_t *count, double *mean, double *sigma,
                          hipStream_t s);
 hipError_t launch_median5(const float *x, uint32_t ndms, uint32_t n, int64_t stride,
-                          const float *dmv, double thr, mdsm_cand *cands, uint32_t cap,
+                          const float *dmv, double thr, int mode, const float *cmean,
+                          const float *crms, uint32_t spc, mdsm_cand *cands, uint32_t cap,
                           uint32_t *count, double *stats, hipStream_t s);
 
 }  // namespace mdsm

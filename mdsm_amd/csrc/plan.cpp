@@ -132,14 +132,25 @@ int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
         b.src = s.src;
         b.dst = s.dst;
         b.dtype = s.dtype;
+        {
+            const int rb = lds_row_bytes(s.dtype, b.W);
+            b.ccr = std::max(1, std::min(kCC, kLdsBudget / rb));
+            st.lds_bytes = std::max(st.lds_bytes, b.ccr * rb);
+        }
         b.tab_off = (int32_t)p->rel.size();
         b.base_off = (int32_t)p->abase.size();
         if (p->rel.size() + (size_t)s.nrows * kDB > (size_t)std::numeric_limits<int32_t>::max())
             return fail(MDSM_ERR_INVALID, "plan shift tables exceed 2^31 entries");
+        // Offsets are stored as byte offsets within the block's LDS row, so the
+        // kernel adds them to a lane address without further arithmetic.
+        const bool intpath = s.dtype == DT_U8 || s.dtype == DT_U16;
+        const int K = b.W / 4;
         for (int i = 0; i < s.nrows; ++i) {
             p->abase.push_back(base[i]);
-            for (int j = 0; j < kDB; ++j)
-                p->rel.push_back(j < nt ? s.sh(j0 + j, i) - base[i] : 0);
+            for (int j = 0; j < kDB; ++j) {
+                const int32_t o = j < nt ? s.sh(j0 + j, i) - base[i] : 0;
+                p->rel.push_back(intpath ? (o & 3) * K * 8 + (o >> 2) * 8 : o * 4);
+            }
         }
         st.wmax = std::max(st.wmax, b.W);
         p->blocks.push_back(b);
@@ -160,7 +171,7 @@ int add_decimation(mdsm_plan *p, int k, uint64_t &work)
     DecimStep d;
     d.k = k;
     d.dtype_in = p->obs.dtype;
-    d.dtype_out = p->obs.dtype == MDSM_U8 ? DT_U16 : DT_F32;
+    d.dtype_out = (p->obs.dtype == MDSM_U8 && k <= 16) ? DT_U16 : DT_F32;
     d.nsamp_out = (int32_t)(p->obs.nsamp_in / (uint32_t)k);
     d.stride = round_up(d.nsamp_out, 64);
     d.work_off = (int64_t)round_up((int64_t)work, 256);
@@ -182,12 +193,19 @@ int upload(mdsm_plan *p)
     };
     if ((e = up((void **)&p->dev.blocks, p->blocks.data(), p->blocks.size() * sizeof(BlockDesc))) !=
             hipSuccess ||
-        (e = up((void **)&p->dev.segs, p->segs.data(), p->segs.size() * sizeof(SegDesc))) !=
-            hipSuccess ||
         (e = up((void **)&p->dev.rel, p->rel.data(), p->rel.size() * sizeof(int32_t))) !=
             hipSuccess ||
         (e = up((void **)&p->dev.abase, p->abase.data(), p->abase.size() * sizeof(int32_t))) !=
             hipSuccess)
+        return hip_fail(e, "uploading plan tables");
+    for (SegDesc &sg : p->segs) {
+        sg.blocks = p->dev.blocks;
+        sg.rel = p->dev.rel;
+        sg.abase = p->dev.abase;
+        sg.job = 0;
+    }
+    if ((e = up((void **)&p->dev.segs, p->segs.data(), p->segs.size() * sizeof(SegDesc))) !=
+        hipSuccess)
         return hip_fail(e, "uploading plan tables");
     return MDSM_OK;
 }
