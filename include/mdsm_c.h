@@ -230,6 +230,92 @@ int mdsm_detect_median5(const float *d_x, uint32_t ndms, uint32_t n,
                         double *d_stats, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Streaming server: resident beams, continuous batching               */
+/* ------------------------------------------------------------------ */
+/* Replaces the reference's per-GPU worker loop and barrier protocol
+ * (src/mdsm/dedispersion_thread.cu:246-401, src/mdsm/dedispersion_manager.cpp:
+ * 324-501) and the host-side overlap copy (dedispersion_thread.cu:266-303):
+ * each beam keeps its max-delay overlap on the device; blocks from many beams
+ * are admitted by HBM budget and dedispersed together in one launch. */
+
+typedef struct mdsm_server mdsm_server;
+
+typedef struct {
+    uint64_t hbm_budget;    /* device bytes the server may hold (0: 90% of free memory) */
+    uint32_t max_inflight;  /* blocks on the device at once (0: 16) */
+    uint32_t max_batch;     /* blocks per dedispersion launch (0: 16; <= 32) */
+    uint32_t persistent;    /* 1: persistent workgroups pulling from a device work counter */
+    uint32_t queue_depth;   /* pushed-but-unadmitted blocks per beam (0: 4) */
+} mdsm_server_config;
+
+typedef struct {
+    float clip_nsigma;      /* > 0: median/MAD clip of each block's new samples (uint8) */
+    double snr_thr;         /* boxcar S/N threshold */
+    uint32_t widths_mask;   /* OR of boxcar widths (powers of two <= 1024); 0: no detection */
+    uint32_t cand_cap;      /* candidates kept per block (0: 65536) */
+    uint32_t retain_output; /* 1: keep each block's DM x time output until mdsm_server_release */
+    uint32_t pad;
+} mdsm_detect_config;
+
+typedef struct {
+    uint32_t beam;
+    int32_t status;         /* MDSM_OK, or the error that ended the block */
+    uint64_t seq;           /* block number within the beam */
+    uint32_t ncand;         /* candidates found (may exceed those returned) */
+    uint32_t nclipped;      /* samples replaced by the clipper */
+    uint32_t batch;         /* blocks in the launch that carried this block */
+    uint32_t pad;
+    double t_arrival;       /* seconds, steady clock (as passed to push, or push time) */
+    double t_admit;         /* admitted to the device */
+    double t_done;          /* results on the host */
+} mdsm_result;
+
+typedef struct {
+    uint64_t pushed, admitted, completed, deferred_budget, deferred_inflight, launches;
+    uint64_t bytes_in_use, bytes_peak, budget;
+    uint32_t beams, inflight;
+} mdsm_server_stats;
+
+int mdsm_server_create(const mdsm_server_config *cfg, mdsm_server **out);
+void mdsm_server_destroy(mdsm_server *srv);
+
+/* Open a beam.  obs->nsamp_in is N, the new samples per block.  The beam's
+ * overlap is M = the plan's maxshift (full-rate samples); output sample t of
+ * block k (range r, decimation k_r) starts at stream sample k*N - M + t*k_r.
+ * Detection runs on the first N/k_r samples of every range row.
+ * MDSM_ERR_BUDGET when the beam's resident buffers plus one block can never
+ * fit the budget. */
+int mdsm_beam_open(mdsm_server *srv, const mdsm_obs *obs, const mdsm_range *ranges,
+                   uint32_t nranges, const mdsm_detect_config *det, uint32_t *beam_id);
+int mdsm_beam_close(mdsm_server *srv, uint32_t beam_id);
+/* overlap M, plan info of the beam (nsamp_in of the plan is M + N) */
+int mdsm_beam_info(mdsm_server *srv, uint32_t beam_id, uint32_t *overlap,
+                   mdsm_plan_info *info);
+
+/* Queue one block: host channel-major [nchans][stride] samples of obs.dtype,
+ * N per channel; copied before return.  arrival_s < 0: stamped now.
+ * MDSM_ERR_AGAIN when the beam's queue is full (back-pressure). */
+int mdsm_beam_push(mdsm_server *srv, uint32_t beam_id, const void *host, int64_t stride,
+                   double arrival_s, uint64_t *seq);
+
+/* Next completed block.  Its candidates (sorted by dm, width, t) are copied
+ * into cands[0..min(ncand,cap)).  wait_ms < 0 waits indefinitely.
+ * MDSM_ERR_AGAIN when nothing completed in time. */
+int mdsm_server_poll(mdsm_server *srv, mdsm_result *res, mdsm_cand *cands, uint32_t cap,
+                     int32_t wait_ms);
+
+/* Retained output of block (beam, seq) (retain_output = 1): device pointer to
+ * the packed plan output, and its release back to the pool. */
+int mdsm_server_output(mdsm_server *srv, uint32_t beam_id, uint64_t seq, const float **d_out);
+int mdsm_server_release(mdsm_server *srv, uint32_t beam_id, uint64_t seq);
+
+/* Block until every pushed block has completed (results stay queued). */
+int mdsm_server_drain(mdsm_server *srv);
+int mdsm_server_get_stats(mdsm_server *srv, mdsm_server_stats *st);
+/* Steady clock in seconds (the clock of t_arrival/t_admit/t_done). */
+double mdsm_now(void);
+
+/* ------------------------------------------------------------------ */
 /* Synthetic filterbanks (seeded, host side)                           */
 /* ------------------------------------------------------------------ */
 

@@ -1,0 +1,839 @@
+// Streaming server: device-resident beams, HBM-budget admission and
+// continuous batching of blocks from many beams into one dedispersion launch.
+//
+// Reference counterpart: the per-GPU worker loop and barrier protocol of
+// src/mdsm/dedispersion_thread.cu:246-401 and src/mdsm/dedispersion_manager.cpp
+// :324-501, which re-upload every block with a host-side overlap copy
+// (dedispersion_thread.cu:266-303) and process one block at a time.
+//
+// Here a beam owns a device ring [C][M+N] whose first M samples per channel
+// are the previous block's tail.  A block needs its beam's previous block to
+// have finished (the overlap), so each beam has at most one block on the
+// device; concurrency comes from many beams.  One scheduler thread admits
+// blocks while transient buffers fit the HBM budget, launches the batch on
+// one of two streams (H2D -> clip -> decimate -> one dedispersion launch for
+// all blocks -> tail save -> boxcar detection -> D2H of candidates) and
+// retires completed batches.  Every block's computation is a function of its
+// own beam's data and tables only, so results do not depend on what it was
+// batched with.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <thread>
+#include <vector>
+
+#include "mdsm_internal.h"
+
+using namespace mdsm;
+
+extern "C" double mdsm_now(void)
+{
+    using namespace std::chrono;
+    return duration<double>(steady_clock::now().time_since_epoch()).count();
+}
+
+namespace {
+
+constexpr int kStreams = 2;
+inline uint64_t up256(uint64_t v) { return (v + 255) / 256 * 256; }
+
+// Budget-capped caching device allocator.  Cached blocks count as used.
+struct DevPool {
+    uint64_t budget = 0, used = 0, peak = 0;
+    std::multimap<uint64_t, void *> cache;
+
+    // *held receives the size class the pointer carries (pass it to release)
+    void *alloc(uint64_t bytes, uint64_t *held)
+    {
+        bytes = up256(bytes);
+        auto it = cache.lower_bound(bytes);
+        if (it != cache.end() && it->first <= bytes * 2) {
+            void *p = it->second;
+            *held = it->first;
+            cache.erase(it);
+            return p;
+        }
+        while (used + bytes > budget && !cache.empty()) {
+            auto big = std::prev(cache.end());
+            (void)hipFree(big->second);
+            used -= big->first;
+            cache.erase(big);
+        }
+        if (used + bytes > budget) return nullptr;
+        void *p = nullptr;
+        if (hipMalloc(&p, bytes) != hipSuccess) return nullptr;
+        used += bytes;
+        peak = std::max(peak, used);
+        *held = bytes;
+        return p;
+    }
+    void release(void *p, uint64_t held)
+    {
+        if (p) cache.emplace(held, p);
+    }
+    void clear()
+    {
+        for (auto &kv : cache) {
+            (void)hipFree(kv.second);
+            used -= kv.first;
+        }
+        cache.clear();
+    }
+};
+
+struct Pending {
+    uint64_t seq;
+    double t_arrival;
+    void *staging;
+};
+
+struct Beam;
+
+struct Job {
+    Beam *beam = nullptr;
+    uint64_t seq = 0;
+    double t_arrival = 0, t_admit = 0;
+    void *staging = nullptr;
+    void *buf = nullptr;     // transient device allocation
+    uint64_t held = 0;       // its size class
+    float *out = nullptr;
+    char *work = nullptr;
+    mdsm_cand *cands = nullptr;
+    uint32_t *count = nullptr;
+    double *mean = nullptr, *sigma = nullptr;
+    int32_t *ncl = nullptr;
+    int status = MDSM_OK;
+    uint32_t batch = 0;
+};
+
+struct Beam {
+    uint32_t id = 0;
+    mdsm_obs obs{};
+    uint32_t N = 0, M = 0;
+    mdsm_plan *plan = nullptr;
+    mdsm_detect_config det{};
+    uint32_t wmask = 0;
+    void *ring = nullptr, *tail = nullptr;
+    uint64_t resident = 0;
+    size_t esize = 1;
+    std::deque<Pending> queue;
+    std::vector<void *> free_staging;
+    std::vector<void *> all_staging;
+    bool busy = false;
+    bool closing = false;
+    uint64_t next_seq = 0;
+    uint64_t trans_bytes = 0;   // transient bytes per block
+    uint64_t off_out = 0, off_work = 0, off_cands = 0, off_count = 0, off_mean = 0,
+             off_sigma = 0, off_ncl = 0;
+    // host result staging (pinned)
+    mdsm_cand *h_cands = nullptr;
+    uint32_t *h_count = nullptr;
+    int32_t *h_ncl = nullptr;
+    std::map<uint64_t, Job *> retained;
+};
+
+struct Batch {
+    std::vector<Job *> jobs;
+    hipEvent_t done = nullptr;
+    int stream = 0;
+    void *segs_dev = nullptr;
+    uint64_t segs_held = 0;
+    std::vector<SegDesc> segs_host;
+};
+
+struct Done {
+    mdsm_result res;
+    std::vector<mdsm_cand> cands;
+};
+
+}  // namespace
+
+struct mdsm_server {
+    mdsm_server_config cfg{};
+    int device = 0;
+    DevPool pool;
+    std::mutex mu;
+    std::condition_variable cv_work, cv_done;
+    std::map<uint32_t, Beam *> beams;
+    uint32_t next_id = 1, rr = 0;
+    std::deque<Batch *> inflight;
+    uint32_t inflight_blocks = 0;
+    std::deque<Done> done;
+    std::thread sched;
+    bool stop = false;
+    hipStream_t streams[kStreams] = {nullptr, nullptr};
+    uint32_t *counters = nullptr;  // persistent-mode work counters, one per stream
+    int next_stream = 0;
+    mdsm_server_stats st{};
+    std::string sched_error;
+};
+
+namespace {
+
+int beam_transient_layout(Beam *b, uint32_t cap)
+{
+    const mdsm_plan_info &pi = b->plan->info;
+    const uint32_t nr = pi.nranges;
+    uint64_t o = 0;
+    b->off_out = o;
+    o = up256(o + pi.out_elems * 4);
+    b->off_work = o;
+    o = up256(o + pi.work_bytes);
+    b->off_cands = o;
+    o = up256(o + (uint64_t)nr * cap * sizeof(mdsm_cand));
+    b->off_count = o;
+    o = up256(o + nr * 4);
+    b->off_mean = o;
+    o = up256(o + (uint64_t)pi.ndms_total * 8);
+    b->off_sigma = o;
+    o = up256(o + (uint64_t)pi.ndms_total * 8);
+    b->off_ncl = o;
+    o = up256(o + (uint64_t)b->obs.nchans * 4);
+    b->trans_bytes = o;
+    return MDSM_OK;
+}
+
+void finish_job(mdsm_server *s, Job *j)
+{
+    Beam *b = j->beam;
+    const uint32_t nr = b->plan->info.nranges;
+    const uint32_t cap = b->det.cand_cap;
+    Done d{};
+    d.res.beam = b->id;
+    d.res.seq = j->seq;
+    d.res.status = j->status;
+    d.res.batch = j->batch;
+    d.res.t_arrival = j->t_arrival;
+    d.res.t_admit = j->t_admit;
+    uint32_t total = 0;
+    if (j->status == MDSM_OK && b->wmask) {
+        for (uint32_t r = 0; r < nr; ++r) {
+            const uint32_t n = b->h_count[r];
+            total += n;
+            const uint32_t first = b->plan->ranges[r].info.first_dm;
+            for (uint32_t i = 0; i < std::min(n, cap); ++i) {
+                mdsm_cand c = b->h_cands[(size_t)r * cap + i];
+                c.dm += first;
+                d.cands.push_back(c);
+            }
+        }
+        std::sort(d.cands.begin(), d.cands.end(), [](const mdsm_cand &a, const mdsm_cand &c) {
+            if (a.dm != c.dm) return a.dm < c.dm;
+            if (a.width != c.width) return a.width < c.width;
+            return a.t < c.t;
+        });
+    }
+    d.res.ncand = total;
+    if (j->status == MDSM_OK && b->det.clip_nsigma > 0.0f && b->obs.dtype == MDSM_U8) {
+        uint64_t ncl = 0;
+        for (uint32_t c = 0; c < b->obs.nchans; ++c) ncl += (uint32_t)b->h_ncl[c];
+        d.res.nclipped = (uint32_t)std::min<uint64_t>(ncl, 0xffffffffu);
+    }
+    d.res.t_done = mdsm_now();
+    b->free_staging.push_back(j->staging);
+    b->busy = false;
+    if (b->det.retain_output && j->status == MDSM_OK) {
+        b->retained[j->seq] = j;
+    } else {
+        s->pool.release(j->buf, j->held);
+        delete j;
+    }
+    s->done.push_back(std::move(d));
+    s->st.completed++;
+}
+
+// Enqueue one batch on a stream.  Called without the lock held; the jobs'
+// beams are busy, so nothing else touches their rings.
+int launch_batch(mdsm_server *s, Batch *bt)
+{
+    hipStream_t st = s->streams[bt->stream];
+    // 1. copy-in, clip, decimate
+    for (Job *j : bt->jobs) {
+        Beam *b = j->beam;
+        const size_t C = b->obs.nchans, M = b->M, N = b->N, es = b->esize;
+        const size_t S = M + N;
+        hipError_t e = hipMemcpy2DAsync((char *)b->ring + M * es, S * es, j->staging, N * es,
+                                        N * es, C, hipMemcpyHostToDevice, st);
+        if (e != hipSuccess) return hip_fail(e, "server H2D");
+        if (b->det.clip_nsigma > 0.0f && b->obs.dtype == MDSM_U8) {
+            TimedLaunch tl("clip", st);
+            e = launch_clip_u8((const uint8_t *)b->ring, (int64_t)S, (uint32_t)C, (int64_t)M,
+                               (uint32_t)N, b->det.clip_nsigma, (uint8_t *)b->ring, (int64_t)S,
+                               nullptr, nullptr, j->ncl, st);
+            if (e != hipSuccess) return hip_fail(e, "server clip");
+        }
+        for (const DecimStep &d : b->plan->decim) {
+            TimedLaunch tl("decimate", st);
+            e = launch_decimate(b->ring, d.dtype_in, (uint32_t)C, (int64_t)S, (uint32_t)S,
+                                (uint32_t)d.k, j->work + d.work_off, d.dtype_out, d.stride, st);
+            if (e != hipSuccess) return hip_fail(e, "server decimate");
+        }
+    }
+    // 2. one dedispersion launch per stage for the whole batch
+    size_t nstages = 0;
+    for (Job *j : bt->jobs) nstages = std::max(nstages, j->beam->plan->stages.size());
+    size_t seg_total = 0;
+    for (Job *j : bt->jobs)
+        for (const Stage &sg : j->beam->plan->stages) seg_total += sg.seg_end - sg.seg_begin;
+    bt->segs_host.clear();
+    bt->segs_host.reserve(seg_total);
+    std::vector<LaunchSpec> specs(nstages);
+    for (size_t k = 0; k < nstages; ++k) {
+        LaunchSpec &L = specs[k];
+        L.seg_begin = (int32_t)bt->segs_host.size();
+        // longest items first, for the persistent work queue
+        std::vector<std::pair<double, const SegDesc *>> order;
+        std::vector<int> jobof;
+        for (size_t ji = 0; ji < bt->jobs.size(); ++ji) {
+            const mdsm_plan *p = bt->jobs[ji]->beam->plan;
+            if (k >= p->stages.size()) continue;
+            const Stage &sg = p->stages[k];
+            for (int i = sg.seg_begin; i < sg.seg_end; ++i) {
+                const SegDesc &d = p->segs[i];
+                const BlockDesc &b0 = p->blocks[d.first_block];
+                order.push_back({-(double)b0.nrows * b0.ntrials, &d});
+                jobof.push_back((int)ji);
+            }
+            L.lds_bytes = std::max(L.lds_bytes, sg.lds_bytes);
+        }
+        std::vector<size_t> idx(order.size());
+        for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
+        std::stable_sort(idx.begin(), idx.end(),
+                         [&](size_t a, size_t c) { return order[a].first < order[c].first; });
+        int64_t items = 0;
+        for (size_t i : idx) {
+            SegDesc d = *order[i].second;
+            d.job = jobof[i];
+            d.item_start = items;
+            items += (int64_t)d.nblocks * d.ntiles;
+            bt->segs_host.push_back(d);
+        }
+        L.nseg = (int32_t)bt->segs_host.size() - L.seg_begin;
+        L.items = items;
+        L.njobs = (int32_t)bt->jobs.size();
+        for (size_t ji = 0; ji < bt->jobs.size(); ++ji) {
+            Job *j = bt->jobs[ji];
+            L.jobs[ji] = JobBufs{j->beam->ring, j->out, j->work, (int64_t)(j->beam->M + j->beam->N)};
+        }
+        if (s->cfg.persistent) {
+            L.counter = s->counters + bt->stream;
+            int dev = 0, cus = 256;
+            if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                hipSuccess)
+                cus = 256;
+            L.grid = cus * 4;
+        }
+    }
+    hipError_t e = hipMemcpyAsync(bt->segs_dev, bt->segs_host.data(),
+                                  bt->segs_host.size() * sizeof(SegDesc), hipMemcpyHostToDevice, st);
+    if (e != hipSuccess) return hip_fail(e, "server segment upload");
+    for (LaunchSpec &L : specs) {
+        L.segs = (const SegDesc *)bt->segs_dev;
+        TimedLaunch tl("dedisperse", st);
+        e = launch_dedisperse_spec(L, st);
+        if (e != hipSuccess) return hip_fail(e, "server dedisperse");
+        s->st.launches++;
+    }
+    // 3. overlap for the next block, detection, results back
+    for (Job *j : bt->jobs) {
+        Beam *b = j->beam;
+        const size_t C = b->obs.nchans, M = b->M, N = b->N, es = b->esize, S = M + N;
+        if (M) {
+            if (N >= M) {
+                e = hipMemcpy2DAsync(b->ring, S * es, (char *)b->ring + N * es, S * es, M * es, C,
+                                     hipMemcpyDeviceToDevice, st);
+            } else {
+                e = hipMemcpy2DAsync(b->tail, M * es, (char *)b->ring + N * es, S * es, M * es, C,
+                                     hipMemcpyDeviceToDevice, st);
+                if (e == hipSuccess)
+                    e = hipMemcpy2DAsync(b->ring, S * es, b->tail, M * es, M * es, C,
+                                         hipMemcpyDeviceToDevice, st);
+            }
+            if (e != hipSuccess) return hip_fail(e, "server overlap save");
+        }
+        const uint32_t nr = b->plan->info.nranges;
+        const uint32_t cap = b->det.cand_cap;
+        if (b->wmask) {
+            SqrtW sw;
+            for (int i = 0; i < 11; ++i) sw.v[i] = std::sqrt((double)(1u << i));
+            for (uint32_t r = 0; r < nr; ++r) {
+                const mdsm_range_info &ri = b->plan->ranges[r].info;
+                const uint32_t n = std::min<uint32_t>(b->N / ri.downsample, ri.nout);
+                TimedLaunch tl("detect", st);
+                e = launch_boxcar(j->out + ri.out_offset, ri.ndms, n, ri.nout, b->wmask,
+                                  b->det.snr_thr, sw, j->cands + (size_t)r * cap, cap,
+                                  j->count + r, j->mean + ri.first_dm, j->sigma + ri.first_dm, st);
+                if (e != hipSuccess) return hip_fail(e, "server detect");
+            }
+            e = hipMemcpyAsync(b->h_count, j->count, nr * 4, hipMemcpyDeviceToHost, st);
+            if (e == hipSuccess)
+                e = hipMemcpyAsync(b->h_cands, j->cands, (size_t)nr * cap * sizeof(mdsm_cand),
+                                   hipMemcpyDeviceToHost, st);
+            if (e != hipSuccess) return hip_fail(e, "server D2H");
+        }
+        if (b->det.clip_nsigma > 0.0f && b->obs.dtype == MDSM_U8) {
+            e = hipMemcpyAsync(b->h_ncl, j->ncl, C * 4, hipMemcpyDeviceToHost, st);
+            if (e != hipSuccess) return hip_fail(e, "server D2H");
+        }
+    }
+    e = hipEventRecord(bt->done, st);
+    if (e != hipSuccess) return hip_fail(e, "server event");
+    return MDSM_OK;
+}
+
+void scheduler(mdsm_server *s)
+{
+    (void)hipSetDevice(s->device);
+    std::unique_lock<std::mutex> lk(s->mu);
+    while (!s->stop) {
+        bool progress = false;
+        // retire completed batches (in order per stream, any order overall)
+        for (auto it = s->inflight.begin(); it != s->inflight.end();) {
+            Batch *bt = *it;
+            hipError_t q = hipEventQuery(bt->done);
+            if (q == hipErrorNotReady) {
+                ++it;
+                continue;
+            }
+            if (q != hipSuccess)
+                for (Job *j : bt->jobs) j->status = MDSM_ERR_DEVICE;
+            for (Job *j : bt->jobs) finish_job(s, j);
+            s->inflight_blocks -= (uint32_t)bt->jobs.size();
+            s->pool.release(bt->segs_dev, bt->segs_held);
+            (void)hipEventDestroy(bt->done);
+            delete bt;
+            it = s->inflight.erase(it);
+            progress = true;
+        }
+        if (progress) s->cv_done.notify_all();
+        // admit a batch
+        Batch *bt = nullptr;
+        if (s->inflight.size() < (size_t)kStreams * 2) {
+            std::vector<Beam *> order;
+            for (auto &kv : s->beams) order.push_back(kv.second);
+            const size_t nb = order.size();
+            for (size_t i = 0; i < nb; ++i) {
+                Beam *b = order[(s->rr + i) % nb];
+                if (b->busy || b->queue.empty()) continue;
+                if (!bt) {
+                    bt = new Batch();
+                    bt->stream = s->next_stream;
+                }
+                if (bt->jobs.size() >= s->cfg.max_batch) break;
+                if (s->inflight_blocks + bt->jobs.size() >= s->cfg.max_inflight) {
+                    s->st.deferred_inflight++;
+                    break;
+                }
+                uint64_t held = 0;
+                void *buf = s->pool.alloc(b->trans_bytes, &held);
+                if (!buf) {
+                    s->st.deferred_budget++;
+                    break;
+                }
+                Pending pd = b->queue.front();
+                b->queue.pop_front();
+                Job *j = new Job();
+                j->beam = b;
+                j->seq = pd.seq;
+                j->t_arrival = pd.t_arrival;
+                j->t_admit = mdsm_now();
+                j->staging = pd.staging;
+                j->buf = buf;
+                j->held = held;
+                char *base = (char *)buf;
+                j->out = (float *)(base + b->off_out);
+                j->work = base + b->off_work;
+                j->cands = (mdsm_cand *)(base + b->off_cands);
+                j->count = (uint32_t *)(base + b->off_count);
+                j->mean = (double *)(base + b->off_mean);
+                j->sigma = (double *)(base + b->off_sigma);
+                j->ncl = (int32_t *)(base + b->off_ncl);
+                b->busy = true;
+                bt->jobs.push_back(j);
+                s->st.admitted++;
+            }
+            s->rr = nb ? (s->rr + 1) % nb : 0;
+            if (bt && bt->jobs.empty()) {
+                delete bt;
+                bt = nullptr;
+            }
+        }
+        if (bt) {
+            size_t nseg = 0;
+            for (Job *j : bt->jobs) nseg += j->beam->plan->segs.size();
+            bt->segs_dev = s->pool.alloc(nseg * sizeof(SegDesc), &bt->segs_held);
+            for (Job *j : bt->jobs) j->batch = (uint32_t)bt->jobs.size();
+            int rc = bt->segs_dev ? MDSM_OK : MDSM_ERR_BUDGET;
+            if (rc == MDSM_OK && hipEventCreateWithFlags(&bt->done, hipEventDisableTiming) != hipSuccess)
+                rc = MDSM_ERR_DEVICE;
+            if (rc == MDSM_OK) {
+                s->next_stream = (s->next_stream + 1) % kStreams;
+                s->inflight_blocks += (uint32_t)bt->jobs.size();
+                lk.unlock();
+                rc = launch_batch(s, bt);
+                lk.lock();
+                if (rc == MDSM_OK) {
+                    s->inflight.push_back(bt);
+                    progress = true;
+                } else {
+                    s->inflight_blocks -= (uint32_t)bt->jobs.size();
+                }
+            }
+            if (rc != MDSM_OK) {
+                // fail the blocks of this batch; the server stays usable
+                if (bt->done) {
+                    (void)hipStreamSynchronize(s->streams[bt->stream]);
+                    (void)hipEventDestroy(bt->done);
+                }
+                for (Job *j : bt->jobs) {
+                    j->status = rc;
+                    finish_job(s, j);
+                }
+                s->pool.release(bt->segs_dev, bt->segs_held);
+                delete bt;
+                s->cv_done.notify_all();
+            }
+        }
+        if (!progress) {
+            // nothing to do right now: wait briefly for pushes or the GPU
+            s->cv_work.wait_for(lk, std::chrono::microseconds(s->inflight.empty() ? 2000 : 100));
+        }
+    }
+}
+
+int upload_and_alloc_beam(mdsm_server *s, Beam *b)
+{
+    int rc = upload_plan(b->plan);
+    if (rc) return rc;
+    const size_t C = b->obs.nchans, S = (size_t)b->M + b->N;
+    uint64_t ring_bytes = up256(C * S * b->esize);
+    uint64_t tail_bytes = (b->N < b->M) ? up256(C * b->M * b->esize) : 0;
+    uint64_t tables = b->plan->blocks.size() * sizeof(BlockDesc) +
+                      b->plan->rel.size() * 4 + b->plan->abase.size() * 4 +
+                      b->plan->segs.size() * sizeof(SegDesc);
+    b->resident = ring_bytes + tail_bytes + up256(tables);
+    if (s->pool.used + b->resident + up256(b->trans_bytes) > s->pool.budget) {
+        s->pool.clear();
+        if (s->pool.used + b->resident + up256(b->trans_bytes) > s->pool.budget)
+            return fail(MDSM_ERR_BUDGET, "beam needs " + std::to_string(b->resident) +
+                                             " resident + " + std::to_string(b->trans_bytes) +
+                                             " per-block bytes; budget " +
+                                             std::to_string(s->pool.budget) + ", in use " +
+                                             std::to_string(s->pool.used));
+    }
+    hipError_t e = hipMalloc(&b->ring, ring_bytes);
+    if (e != hipSuccess) return hip_fail(e, "beam ring");
+    e = hipMemset(b->ring, 0, ring_bytes);  // block 0 sees a zero history
+    if (e != hipSuccess) return hip_fail(e, "beam ring clear");
+    if (tail_bytes) {
+        e = hipMalloc(&b->tail, tail_bytes);
+        if (e != hipSuccess) return hip_fail(e, "beam tail");
+    }
+    s->pool.used += b->resident;
+    s->pool.peak = std::max(s->pool.peak, s->pool.used);
+    const uint32_t nr = b->plan->info.nranges;
+    if (hipHostMalloc((void **)&b->h_cands, (size_t)nr * b->det.cand_cap * sizeof(mdsm_cand)) !=
+            hipSuccess ||
+        hipHostMalloc((void **)&b->h_count, nr * 4) != hipSuccess ||
+        hipHostMalloc((void **)&b->h_ncl, C * 4) != hipSuccess)
+        return fail(MDSM_ERR_NOMEM, "beam host staging");
+    const size_t block_bytes = C * b->N * b->esize;
+    for (uint32_t i = 0; i < s->cfg.queue_depth + 1; ++i) {
+        void *p = nullptr;
+        if (hipHostMalloc(&p, block_bytes) != hipSuccess)
+            return fail(MDSM_ERR_NOMEM, "beam block staging");
+        b->all_staging.push_back(p);
+        b->free_staging.push_back(p);
+    }
+    return MDSM_OK;
+}
+
+void free_beam(mdsm_server *s, Beam *b)
+{
+    for (auto &kv : b->retained) {
+        s->pool.release(kv.second->buf, kv.second->held);
+        delete kv.second;
+    }
+    b->retained.clear();
+    if (b->ring) {
+        (void)hipFree(b->ring);
+        s->pool.used -= b->resident;
+    }
+    if (b->tail) (void)hipFree(b->tail);
+    for (void *p : b->all_staging) (void)hipHostFree(p);
+    if (b->h_cands) (void)hipHostFree(b->h_cands);
+    if (b->h_count) (void)hipHostFree(b->h_count);
+    if (b->h_ncl) (void)hipHostFree(b->h_ncl);
+    if (b->plan) mdsm_plan_destroy(b->plan);
+    delete b;
+}
+
+}  // namespace
+
+extern "C" int mdsm_server_create(const mdsm_server_config *cfg, mdsm_server **out)
+{
+    if (!out) return fail(MDSM_ERR_INVALID, "out is NULL");
+    *out = nullptr;
+    auto *s = new mdsm_server();
+    if (cfg) s->cfg = *cfg;
+    if (!s->cfg.max_inflight) s->cfg.max_inflight = 16;
+    if (!s->cfg.max_batch) s->cfg.max_batch = 16;
+    s->cfg.max_batch = std::min<uint32_t>(s->cfg.max_batch, kMaxJobs);
+    if (!s->cfg.queue_depth) s->cfg.queue_depth = 4;
+    hipError_t e = hipGetDevice(&s->device);
+    if (e != hipSuccess) {
+        delete s;
+        return hip_fail(e, "hipGetDevice");
+    }
+    size_t fr = 0, tot = 0;
+    e = hipMemGetInfo(&fr, &tot);
+    if (e != hipSuccess) {
+        delete s;
+        return hip_fail(e, "hipMemGetInfo");
+    }
+    s->pool.budget = s->cfg.hbm_budget ? s->cfg.hbm_budget : (uint64_t)(fr * 0.9);
+    s->st.budget = s->pool.budget;
+    for (auto &st : s->streams) {
+        e = hipStreamCreateWithFlags(&st, hipStreamNonBlocking);
+        if (e != hipSuccess) {
+            mdsm_server_destroy(s);
+            return hip_fail(e, "server stream");
+        }
+    }
+    e = hipMalloc((void **)&s->counters, kStreams * sizeof(uint32_t));
+    if (e != hipSuccess) {
+        mdsm_server_destroy(s);
+        return hip_fail(e, "server counters");
+    }
+    s->sched = std::thread(scheduler, s);
+    *out = s;
+    return MDSM_OK;
+}
+
+extern "C" void mdsm_server_destroy(mdsm_server *s)
+{
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->stop = true;
+    }
+    s->cv_work.notify_all();
+    if (s->sched.joinable()) s->sched.join();
+    for (auto &st : s->streams)
+        if (st) (void)hipStreamSynchronize(st);
+    for (Batch *bt : s->inflight) {
+        for (Job *j : bt->jobs) {
+            s->pool.release(j->buf, j->held);
+            delete j;
+        }
+        s->pool.release(bt->segs_dev, bt->segs_held);
+        if (bt->done) (void)hipEventDestroy(bt->done);
+        delete bt;
+    }
+    for (auto &kv : s->beams) free_beam(s, kv.second);
+    s->pool.clear();
+    if (s->counters) (void)hipFree(s->counters);
+    for (auto &st : s->streams)
+        if (st) (void)hipStreamDestroy(st);
+    delete s;
+}
+
+extern "C" int mdsm_beam_open(mdsm_server *s, const mdsm_obs *obs, const mdsm_range *ranges,
+                              uint32_t nranges, const mdsm_detect_config *det, uint32_t *beam_id)
+{
+    if (!s || !obs || !beam_id) return fail(MDSM_ERR_INVALID, "NULL argument");
+    if (obs->nsamp_in < 1) return fail(MDSM_ERR_INVALID, "block length N must be >= 1");
+    auto *b = new Beam();
+    b->obs = *obs;
+    b->N = obs->nsamp_in;
+    b->esize = obs->dtype == MDSM_U8 ? 1 : (obs->dtype == MDSM_F16 ? 2 : 4);
+    if (det) b->det = *det;
+    if (!b->det.cand_cap) b->det.cand_cap = 65536;
+    b->wmask = b->det.widths_mask;
+    if (b->wmask & ~0x7ffu) {
+        delete b;
+        return fail(MDSM_ERR_INVALID, "boxcar widths must be powers of two <= 1024");
+    }
+    if (b->det.clip_nsigma > 0.0f && obs->dtype != MDSM_U8) {
+        delete b;
+        return fail(MDSM_ERR_INVALID, "the clipper runs on uint8 beams");
+    }
+    // overlap = the plan's maximum delay, rounded to whole decimated samples
+    mdsm_obs o = *obs;
+    o.nsamp_in = 1u << 30;
+    mdsm_plan *probe = nullptr;
+    int rc = mdsm_plan_create(&o, ranges, nranges, &probe);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    uint32_t kmax = 1;
+    for (auto &rh : probe->ranges) kmax = std::max(kmax, rh.info.downsample);
+    uint32_t M = probe->info.maxshift;
+    mdsm_plan_destroy(probe);
+    M = (M + kmax - 1) / kmax * kmax;
+    if (b->N % kmax) {
+        delete b;
+        return fail(MDSM_ERR_INVALID, "block length N must be a multiple of every downsample");
+    }
+    b->M = M;
+    o.nsamp_in = M + b->N;
+    rc = mdsm_plan_create(&o, ranges, nranges, &b->plan);
+    if (rc) {
+        delete b;
+        return rc;
+    }
+    beam_transient_layout(b, b->det.cand_cap);
+    std::lock_guard<std::mutex> g(s->mu);
+    (void)hipSetDevice(s->device);
+    rc = upload_and_alloc_beam(s, b);
+    if (rc) {
+        free_beam(s, b);
+        return rc;
+    }
+    b->id = s->next_id++;
+    s->beams[b->id] = b;
+    s->st.beams = (uint32_t)s->beams.size();
+    *beam_id = b->id;
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_beam_info(mdsm_server *s, uint32_t id, uint32_t *overlap, mdsm_plan_info *info)
+{
+    if (!s) return fail(MDSM_ERR_INVALID, "NULL server");
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->beams.find(id);
+    if (it == s->beams.end()) return fail(MDSM_ERR_INVALID, "no such beam");
+    if (overlap) *overlap = it->second->M;
+    if (info) *info = it->second->plan->info;
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_beam_push(mdsm_server *s, uint32_t id, const void *host, int64_t stride,
+                              double arrival_s, uint64_t *seq)
+{
+    if (!s || !host) return fail(MDSM_ERR_INVALID, "NULL argument");
+    void *stg = nullptr;
+    Beam *b = nullptr;
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->beams.find(id);
+        if (it == s->beams.end()) return fail(MDSM_ERR_INVALID, "no such beam");
+        b = it->second;
+        if (b->closing) return fail(MDSM_ERR_STATE, "beam is closing");
+        if (stride < (int64_t)b->N) return fail(MDSM_ERR_INVALID, "stride < block length");
+        if (b->queue.size() >= s->cfg.queue_depth || b->free_staging.empty())
+            return fail(MDSM_ERR_AGAIN, "beam queue full");
+        stg = b->free_staging.back();
+        b->free_staging.pop_back();
+    }
+    const size_t C = b->obs.nchans, N = b->N, es = b->esize;
+    for (size_t c = 0; c < C; ++c)
+        std::memcpy((char *)stg + c * N * es, (const char *)host + c * (size_t)stride * es, N * es);
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        Pending p{b->next_seq++, arrival_s >= 0 ? arrival_s : mdsm_now(), stg};
+        b->queue.push_back(p);
+        s->st.pushed++;
+        if (seq) *seq = p.seq;
+    }
+    s->cv_work.notify_all();
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_server_poll(mdsm_server *s, mdsm_result *res, mdsm_cand *cands, uint32_t cap,
+                                int32_t wait_ms)
+{
+    if (!s || !res) return fail(MDSM_ERR_INVALID, "NULL argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto ready = [&] { return !s->done.empty(); };
+    if (!ready()) {
+        if (wait_ms < 0)
+            s->cv_done.wait(lk, ready);
+        else if (wait_ms > 0)
+            s->cv_done.wait_for(lk, std::chrono::milliseconds(wait_ms), ready);
+    }
+    if (!ready()) return MDSM_ERR_AGAIN;
+    Done d = std::move(s->done.front());
+    s->done.pop_front();
+    *res = d.res;
+    if (cands && cap)
+        std::memcpy(cands, d.cands.data(), std::min<size_t>(cap, d.cands.size()) * sizeof(mdsm_cand));
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_server_output(mdsm_server *s, uint32_t id, uint64_t seq, const float **d_out)
+{
+    if (!s || !d_out) return fail(MDSM_ERR_INVALID, "NULL argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->beams.find(id);
+    if (it == s->beams.end()) return fail(MDSM_ERR_INVALID, "no such beam");
+    auto r = it->second->retained.find(seq);
+    if (r == it->second->retained.end()) return fail(MDSM_ERR_STATE, "block output not retained");
+    *d_out = r->second->out;
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_server_release(mdsm_server *s, uint32_t id, uint64_t seq)
+{
+    if (!s) return fail(MDSM_ERR_INVALID, "NULL argument");
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->beams.find(id);
+        if (it == s->beams.end()) return fail(MDSM_ERR_INVALID, "no such beam");
+        auto r = it->second->retained.find(seq);
+        if (r == it->second->retained.end()) return fail(MDSM_ERR_STATE, "block output not retained");
+        s->pool.release(r->second->buf, r->second->held);
+        delete r->second;
+        it->second->retained.erase(r);
+    }
+    s->cv_work.notify_all();
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_server_drain(mdsm_server *s)
+{
+    if (!s) return fail(MDSM_ERR_INVALID, "NULL argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    for (;;) {
+        bool idle = s->inflight.empty();
+        for (auto &kv : s->beams) idle = idle && kv.second->queue.empty() && !kv.second->busy;
+        if (idle) return MDSM_OK;
+        s->cv_work.notify_all();
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(5));
+    }
+}
+
+extern "C" int mdsm_beam_close(mdsm_server *s, uint32_t id)
+{
+    if (!s) return fail(MDSM_ERR_INVALID, "NULL argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->beams.find(id);
+    if (it == s->beams.end()) return fail(MDSM_ERR_INVALID, "no such beam");
+    Beam *b = it->second;
+    b->closing = true;
+    while (!b->queue.empty() || b->busy) {
+        s->cv_work.notify_all();
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(5));
+    }
+    s->beams.erase(it);
+    s->st.beams = (uint32_t)s->beams.size();
+    free_beam(s, b);
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_server_get_stats(mdsm_server *s, mdsm_server_stats *st)
+{
+    if (!s || !st) return fail(MDSM_ERR_INVALID, "NULL argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    *st = s->st;
+    st->bytes_in_use = s->pool.used;
+    st->bytes_peak = s->pool.peak;
+    st->inflight = s->inflight_blocks;
+    return MDSM_OK;
+}

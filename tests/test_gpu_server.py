@@ -1,0 +1,163 @@
+"""Serving loop: continuous batching across beams, device-resident overlap,
+clip -> dedisperse -> boxcar per block, HBM-budget admission.
+
+The key property: each block's result (DM x time output and candidate list)
+is the same when it runs alone and when it runs batched with other beams'
+blocks, and equals the oracle pipeline over the concatenated stream."""
+import numpy as np
+import pytest
+
+import oracle
+
+pytestmark = pytest.mark.gpu
+WIDTHS = (1, 2, 4, 8, 16, 32, 64)
+
+
+def _beam_kwargs(C=64, N=4096, ranges=((0.0, 2.0, 40, 1), (80.0, 4.0, 20, 2)), clip=5.0):
+    return dict(nchans=C, fch1=1500.0, foff=-300.0 / C, tsamp=64e-6, nsamp_block=N, dtype="u8",
+                ranges=ranges, clip_nsigma=clip, snr_thr=6.0, widths=WIDTHS,
+                retain_output=True)
+
+
+def _stream(M, C, N, nblocks, seed, pulses):
+    return M.synth_filterbank(C, N * nblocks, "u8", seed=seed, mean=128, sigma=16,
+                              fch1=1500.0, foff=-300.0 / C, pulses=pulses)
+
+
+def _run(srv, beams_and_streams, N, nblocks, interleave=True):
+    """Push every block of every beam; returns {(beam, seq): (out, cands)}."""
+    res = {}
+    pending = 0
+    order = []
+    for k in range(nblocks):
+        for b, x in beams_and_streams:
+            order.append((b, k, x))
+    if not interleave:
+        order.sort(key=lambda t: (t[0].id, t[1]))
+    for b, k, x in order:
+        blk = np.ascontiguousarray(x[:, k * N:(k + 1) * N])
+        while srv.try_push(b, blk) is None:
+            r, c = srv.poll(wait_ms=-1)
+            res[(r.beam, r.seq)] = (srv.output(srv.beams[r.beam], r.seq), c, r)
+            srv.release(srv.beams[r.beam], r.seq)
+            pending -= 1
+        pending += 1
+    srv.drain()
+    while True:
+        r, c = srv.poll(wait_ms=0)
+        if r is None:
+            break
+        assert r.status == 0
+        res[(r.beam, r.seq)] = (srv.output(srv.beams[r.beam], r.seq), c, r)
+        srv.release(srv.beams[r.beam], r.seq)
+    return res
+
+
+def _oracle_beam(x, beam, N, nblocks, ranges, C):
+    """Oracle pipeline over the stream: clip each block's new samples, then
+    dedisperse the zero-history + stream series and detect per block."""
+    M = beam.overlap
+    clipped = np.empty_like(x)
+    for k in range(nblocks):
+        out, *_ = oracle.clip_u8(x[:, k * N:(k + 1) * N], 0, N, 5.0)
+        clipped[:, k * N:(k + 1) * N] = out
+    full = np.concatenate([np.zeros((C, M), np.uint8), clipped], axis=1)
+    per_block = []
+    for k in range(nblocks):
+        seg = np.ascontiguousarray(full[:, k * N:k * N + M + N])
+        outs, cands = [], []
+        first = 0
+        for (lo, step, nd, kk) in ranges:
+            ts = np.float32(64e-6) * np.float32(kk)
+            sh = oracle.shift_table_ref(1500.0, -300.0 / C, C, lo, step, nd, ts)
+            nout = (M + N) // kk - int(sh.max())
+            o = oracle.range_f32(seg, kk, sh, nout)
+            outs.append(o.reshape(-1))
+            c, _, _ = oracle.boxcar_detect(np.ascontiguousarray(o[:, :N // kk]), list(WIDTHS), 6.0)
+            c["dm"] += first
+            cands.append(c)
+            first += nd
+        per_block.append((np.concatenate(outs), np.concatenate(cands)))
+    return per_block
+
+
+def _key(c):
+    return sorted(zip(c["dm"].tolist(), c["t"].tolist(), c["width"].tolist(), c["snr"].tolist()))
+
+
+def test_alone_and_in_mix_identical_and_match_oracle(gpu):
+    from mdsm_amd.server import Server
+    M = gpu
+    C, N, nblocks = 64, 4096, 3
+    ranges = ((0.0, 2.0, 40, 1), (80.0, 4.0, 20, 2))
+    print("case server alone/mix", flush=True)
+    streams = [_stream(M, C, N, nblocks, 100 + i, [(30.0 + 20 * i, 2000 + 3000 * i, 4, 3.0)])
+               for i in range(4)]
+    for x in streams:          # impulsive RFI for the clipper
+        x[5, 100:110] = 255
+    # alone: one server per beam, one beam at a time
+    alone = []
+    for i, x in enumerate(streams):
+        srv = Server(max_batch=1)
+        b = srv.open_beam(**_beam_kwargs(C, N, ranges))
+        r = _run(srv, [(b, x)], N, nblocks)
+        alone.append({k[1]: v for k, v in r.items()})
+        srv.close()
+    # mix: all beams on one server, interleaved pushes, batched launches
+    srv = Server(max_batch=8)
+    bs = [srv.open_beam(**_beam_kwargs(C, N, ranges)) for _ in streams]
+    mix = _run(srv, list(zip(bs, streams)), N, nblocks)
+    batches = [v[2].batch for v in mix.values()]
+    assert max(batches) > 1, "blocks were never batched together"
+    for i, b in enumerate(bs):
+        ref = _oracle_beam(streams[i], b, N, nblocks, ranges, C)
+        for k in range(nblocks):
+            out_m, c_m, _ = mix[(b.id, k)]
+            out_a, c_a, _ = alone[i][k]
+            assert np.array_equal(out_m, out_a), f"beam {i} block {k}: output differs in the mix"
+            assert _key(c_m) == _key(c_a), f"beam {i} block {k}: candidates differ in the mix"
+            assert np.array_equal(out_m, ref[k][0]), f"beam {i} block {k}: output != oracle"
+            assert [t[:3] for t in _key(c_m)] == [t[:3] for t in _key(ref[k][1])], \
+                f"beam {i} block {k}: candidates != oracle"
+    st = srv.stats()
+    assert st["completed"] == len(streams) * nblocks and st["launches"] < st["completed"]
+    srv.close()
+
+
+def test_budget_admission(gpu):
+    """A beam that can never fit is rejected; a budget for one block at a time
+    defers admission but completes everything."""
+    from mdsm_amd.server import Server
+    M = gpu
+    print("case server budget", flush=True)
+    srv = Server(hbm_budget=1 << 20)
+    with pytest.raises(M.MdsmError) as ei:
+        srv.open_beam(**_beam_kwargs())
+    assert ei.value.code == -5
+    srv.close()
+    srv = Server(hbm_budget=(1 << 30), max_inflight=1, max_batch=4)
+    C, N = 64, 4096
+    bs = [srv.open_beam(**_beam_kwargs(C, N)) for _ in range(3)]
+    xs = [_stream(M, C, N, 2, 7 + i, []) for i in range(3)]
+    out = _run(srv, list(zip(bs, xs)), N, 2)
+    assert len(out) == 6
+    st = srv.stats()
+    assert st["deferred_inflight"] > 0 and st["bytes_peak"] <= st["budget"]
+    srv.close()
+
+
+def test_persistent_launch_matches(gpu):
+    from mdsm_amd.server import Server
+    M = gpu
+    print("case server persistent", flush=True)
+    C, N = 64, 4096
+    xs = [_stream(M, C, N, 2, 50 + i, [(40.0, 1500, 8, 4.0)]) for i in range(3)]
+    outs = []
+    for persistent in (False, True):
+        srv = Server(max_batch=8, persistent=persistent)
+        bs = [srv.open_beam(**_beam_kwargs(C, N)) for _ in xs]
+        r = _run(srv, list(zip(bs, xs)), N, 2)
+        outs.append({(i, k): r[(b.id, k)][0] for i, b in enumerate(bs) for k in range(2)})
+        srv.close()
+    for key in outs[0]:
+        assert np.array_equal(outs[0][key], outs[1][key])
