@@ -33,7 +33,124 @@ WORKLOADS = {
                                      (500.0, 1.0, 500, 4)]),
     "cfg1": dict(nchans=256, nsamp=65536, fch1=1500.0, foff=-0.390625, tsamp=64e-6,
                  dtype="f32", ranges=[(0.0, 0.2, 512, 1)]),
+    # serving: 32 beams of 1024-ch uint8 blocks of 2^16 samples, 2048 DMs over 0-1000
+    "cfg5": dict(nchans=1024, nsamp=1 << 16, fch1=1500.0, foff=-300.0 / 1024, tsamp=64e-6,
+                 dtype="u8", ranges=[(0.0, 1000.0 / 2048, 2048, 1)], beams=32,
+                 mean_interval_s=4.194304),
 }
+
+
+def serve(M, w, args):
+    """Config 5: Poisson block arrivals for `beams` beams into the streaming
+    server (clip 5 sigma, 2048-DM dedispersion, boxcar w=1..64 at S/N > 6).
+    Two phases: (1) throughput -- every beam's blocks pushed back to back,
+    real-time factor = seconds of data / wall seconds; (2) latency -- Poisson
+    arrivals, time-compressed by --speedup, latency = candidates ready -
+    arrival per block (p50/p99).  Beside it: the plain single-beam float32
+    kernel on the same block shape."""
+    from mdsm_amd.server import Server, now
+    C, N, B = w["nchans"], w["nsamp"], w["beams"]
+    rng = np.random.default_rng(5)
+    nuniq = 4
+    blocks = []
+    for i in range(nuniq):
+        pulses = [(float(rng.uniform(50, 900)), int(rng.integers(1000, N - 20000)), 8, 1.5)] \
+            if i == 0 else []
+        blocks.append(M.synth_filterbank(C, N, "u8", seed=900 + i, fch1=w["fch1"],
+                                         foff=w["foff"], tsamp=w["tsamp"], mean=128.0,
+                                         sigma=16.0, pulses=pulses, nthreads=16))
+    srv = Server(max_inflight=16, max_batch=8)
+    beams = [srv.open_beam(C, w["fch1"], w["foff"], w["tsamp"], N, "u8", w["ranges"],
+                           clip_nsigma=5.0, snr_thr=6.0, widths=(1, 2, 4, 8, 16, 32, 64),
+                           cand_cap=1 << 16) for _ in range(B)]
+    block_s = N * w["tsamp"]
+    results = []
+
+    def collect(wait_ms):
+        r, c = srv.poll(wait_ms=wait_ms)
+        if r is not None:
+            assert r.status == 0, r.status
+            results.append((r.beam, r.seq, r.t_arrival, r.t_admit, r.t_done, r.batch, r.ncand))
+        return r
+
+    # warm-up: one block per beam
+    for b in beams:
+        while srv.try_push(b, blocks[0]) is None:
+            collect(-1)
+    srv.drain()
+    while collect(0) is not None:
+        pass
+    results.clear()
+    # (1) throughput
+    nb = args.serve_blocks
+    t0 = now()
+    for k in range(nb):
+        for i, b in enumerate(beams):
+            while srv.try_push(b, blocks[(k + i) % nuniq]) is None:
+                collect(-1)
+    srv.drain()
+    wall = now() - t0
+    while collect(0) is not None:
+        pass
+    data_s = nb * B * block_s
+    rt_factor = data_s / wall
+    thr_units = nb * B * float(beams[0].info.dm_chan_samples) / wall
+    # (2) latency under Poisson arrivals, time compressed
+    results.clear()
+    speed = args.speedup
+    events = []
+    for i, b in enumerate(beams):
+        t = rng.exponential(w["mean_interval_s"])
+        for k in range(args.serve_blocks):
+            events.append((t / speed, i, k))
+            t += rng.exponential(w["mean_interval_s"])
+    events.sort()
+    t_start = now()
+    for (ta, i, k) in events:
+        while now() - t_start < ta:
+            collect(1)
+        arr = t_start + ta
+        while srv.try_push(beams[i], blocks[(k + i) % nuniq], arrival=arr) is None:
+            collect(-1)
+    srv.drain()
+    while collect(0) is not None:
+        pass
+    lat = np.array([r[4] - r[2] for r in results])
+    batch = np.array([r[5] for r in results])
+    st = srv.stats()
+    srv.close()
+    # plain single-beam float32 kernel on the same block shape
+    x32 = blocks[0].astype(np.float32)
+    M_over = beams[0].overlap
+    plan = M.Plan.brute(C, w["fch1"], w["foff"], w["tsamp"], N + M_over, "f32", w["ranges"])
+    xin = np.concatenate([np.zeros((C, M_over), np.float32), x32], axis=1)
+    din = M.DeviceBuffer.from_host(xin)
+    dout = M.DeviceBuffer(plan.info.out_elems * 4)
+    plan.dedisperse(din, N + M_over, dout)
+    M.synchronize()
+    t1 = now()
+    reps = 5
+    for _ in range(reps):
+        plan.dedisperse(din, N + M_over, dout)
+    M.synchronize()
+    plain_s = (now() - t1) / reps
+    plain_rate = plan.info.dm_chan_samples / plain_s
+    return {
+        "metric": "DM*chan*samples/s (served, clip+dedisperse+boxcar)",
+        "value": thr_units, "unit": "DM*chan*samples/s", "n_gpus": 1,
+        "higher_is_better": True, "dtype": "u8->f32", "data": "synthetic",
+        "config": {"workload": f"cfg5: {B} beams x 1024 ch u8 blocks of {N} samples, "
+                   "2048 DMs 0-1000, clip 5 sigma, boxcar 1..64 S/N>6",
+                   "blocks_per_beam": nb, "speedup": speed},
+        "rt_factor": rt_factor, "block_seconds": block_s,
+        "latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                      "max": float(lat.max()), "n": int(lat.size)},
+        "mean_batch": float(batch.mean()),
+        "plain_f32_single_beam": {"value": plain_rate, "ms_per_block": plain_s * 1e3},
+        "served_vs_plain": thr_units / plain_rate,
+        "server_stats": {k: int(v) for k, v in st.items()},
+    }
+
 
 
 def make_input(M, w, seed=20240601):
@@ -133,6 +250,9 @@ def main():
     ap.add_argument("--workload", default="cfg2", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--cpu-dms", type=int, default=16)
+    ap.add_argument("--serve-blocks", type=int, default=3, help="cfg5: blocks per beam")
+    ap.add_argument("--speedup", type=float, default=20.0,
+                    help="cfg5: arrival-time compression of the latency phase")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported in this build "
@@ -141,6 +261,10 @@ def main():
         sys.exit(2)
     import mdsm_amd as M
     w = WORKLOADS[args.workload]
+    if args.workload == "cfg5":
+        print("[bench] cfg5 serving", file=sys.stderr, flush=True)
+        print(json.dumps(serve(M, w, args)), flush=True)
+        return
     print(f"[bench] workload {args.workload}: generating input", file=sys.stderr, flush=True)
     x, pulses = make_input(M, w)
     plan = M.Plan.brute(w["nchans"], w["fch1"], w["foff"], w["tsamp"], w["nsamp"], w["dtype"],

@@ -112,86 +112,162 @@ __device__ __forceinline__ uint32_t load_u(const void *base, int64_t i)
         return ((const uint16_t *)base)[i];
 }
 
+// Global fetch of 8 consecutive samples (one LDS pair column) into raw
+// registers: uint8 -> 2 dwords of bytes, uint16 -> 4 dwords of halves.
 template <int DT>
-__device__ __forceinline__ void load_tile_int(uint2 *lds, const void *src, int64_t in_off,
-                                              int64_t row0, int64_t rs, const int32_t *ab, int t0,
-                                              int W, int cc, int in_len, int wave, int lane)
+struct RawWords {
+    static constexpr int n = DT == DT_U8 ? 2 : 4;
+};
+
+template <int DT>
+__device__ __forceinline__ void fetch8(const void *src, int64_t rowbase, int col, int in_len,
+                                       bool aligned, uint32_t *raw)
 {
-    const int K = W >> 2;
-    for (int i = wave; i < cc; i += kWaves) {
-        const int64_t rowbase = in_off + (row0 + i) * rs;
-        const int col0 = ab[i] + t0;
-        uint2 *dst = lds + (size_t)i * 4 * K;
-        const bool aligned = ((rowbase * (DT == DT_U8 ? 1 : 2)) & 3) == 0;
-        for (int k = lane; k < K; k += 64) {
-            const int col = col0 + 4 * k;
-            uint32_t d0[4], d1[4];
-            if (aligned && col + 8 <= in_len) {
-                if constexpr (DT == DT_U8) {
-                    const uint32_t *p = (const uint32_t *)((const uint8_t *)src + rowbase + col);
-                    const uint32_t a = p[0], b = p[1];  // samples 0-3, 4-7
+    constexpr int n = RawWords<DT>::n;
+    constexpr int es = DT == DT_U8 ? 1 : 2;
+    if (aligned && col + 8 <= in_len) {
+        const uint32_t *p = (const uint32_t *)((const char *)src + (rowbase + col) * es);
 #pragma unroll
-                    for (int ph = 0; ph < 4; ++ph) {
-                        d0[ph] = perm(b, a, 0x0c000c00u | (uint32_t)ph | ((uint32_t)(ph + 2) << 16));
-                        d1[ph] = perm(b, a, 0x0c000c00u | (uint32_t)(ph + 1) | ((uint32_t)(ph + 3) << 16));
-                    }
-                } else {
-                    const uint32_t *p = (const uint32_t *)((const uint16_t *)src + rowbase + col);
-                    const uint32_t h0 = p[0], h1 = p[1], h2 = p[2], h3 = p[3];
-                    const uint32_t P0 = perm(h1, h0, 0x05040100u), Q0 = perm(h1, h0, 0x07060302u);
-                    const uint32_t P1 = perm(h2, h1, 0x05040100u), Q1 = perm(h2, h1, 0x07060302u);
-                    const uint32_t P2 = perm(h3, h2, 0x05040100u);
-                    d0[0] = P0; d1[0] = Q0;
-                    d0[1] = Q0; d1[1] = P1;
-                    d0[2] = P1; d1[2] = Q1;
-                    d0[3] = Q1; d1[3] = P2;
-                }
-            } else {
-                uint32_t v[8];
+        for (int i = 0; i < n; ++i) raw[i] = p[i];
+    } else {
 #pragma unroll
-                for (int j = 0; j < 8; ++j)
-                    v[j] = (col + j < in_len) ? load_u<DT>(src, rowbase + col + j) : 0u;
+        for (int i = 0; i < n; ++i) raw[i] = 0u;
 #pragma unroll
-                for (int ph = 0; ph < 4; ++ph) {
-                    d0[ph] = v[ph] | (v[ph + 2] << 16);
-                    d1[ph] = v[ph + 1] | (v[ph + 3] << 16);
-                }
-            }
-#pragma unroll
-            for (int ph = 0; ph < 4; ++ph) dst[ph * K + k] = make_uint2(d0[ph], d1[ph]);
+        for (int j = 0; j < 8; ++j) {
+            const uint32_t v = (col + j < in_len) ? load_u<DT>(src, rowbase + col + j) : 0u;
+            if constexpr (DT == DT_U8)
+                raw[j >> 2] |= v << (8 * (j & 3));
+            else
+                raw[j >> 1] |= v << (16 * (j & 1));
         }
     }
 }
 
-__device__ void dedisperse_int(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
-                               int lane, uint2 *lds)
+// Write the four phases of one pair column k from its 8 raw samples.
+template <int DT>
+__device__ __forceinline__ void store_phases(uint2 *row, int K, int k, const uint32_t *raw)
+{
+    uint32_t d0[4], d1[4];
+    if constexpr (DT == DT_U8) {
+        const uint32_t a = raw[0], b = raw[1];  // samples 0-3, 4-7
+#pragma unroll
+        for (int ph = 0; ph < 4; ++ph) {
+            d0[ph] = perm(b, a, 0x0c000c00u | (uint32_t)ph | ((uint32_t)(ph + 2) << 16));
+            d1[ph] = perm(b, a, 0x0c000c00u | (uint32_t)(ph + 1) | ((uint32_t)(ph + 3) << 16));
+        }
+    } else {
+        const uint32_t h0 = raw[0], h1 = raw[1], h2 = raw[2], h3 = raw[3];
+        const uint32_t P0 = perm(h1, h0, 0x05040100u), Q0 = perm(h1, h0, 0x07060302u);
+        const uint32_t P1 = perm(h2, h1, 0x05040100u), Q1 = perm(h2, h1, 0x07060302u);
+        const uint32_t P2 = perm(h3, h2, 0x05040100u);
+        d0[0] = P0; d1[0] = Q0;
+        d0[1] = Q0; d1[1] = P1;
+        d0[2] = P1; d1[2] = Q1;
+        d0[3] = Q1; d1[3] = P2;
+    }
+#pragma unroll
+    for (int ph = 0; ph < 4; ++ph) row[ph * K + k] = make_uint2(d0[ph], d1[ph]);
+}
+
+// Register prefetch of one chunk: wave w owns rows w and w+4, each up to 4
+// pair columns per lane (K <= 256).
+constexpr int kPfRows = 2, kPfIters = 4;
+
+template <int DT>
+__device__ __forceinline__ void fetch_chunk(uint32_t (&pre)[kPfRows][kPfIters][4], const void *src,
+                                            int64_t in_off, int64_t row0, int64_t rs,
+                                            const int32_t *ab, int t0, int K, int cc, int in_len,
+                                            int wave, int lane)
+{
+    constexpr int es = DT == DT_U8 ? 1 : 2;
+#pragma unroll
+    for (int rr = 0; rr < kPfRows; ++rr) {
+        const int i = wave + kWaves * rr;
+        if (i >= cc) break;
+        const int64_t rowbase = in_off + (row0 + i) * rs;
+        const int col0 = ab[i] + t0;
+        const bool aligned = ((rowbase * es) & 3) == 0;
+#pragma unroll
+        for (int it = 0; it < kPfIters; ++it) {
+            const int k = lane + 64 * it;
+            if (64 * it >= K) break;
+            if (k < K) fetch8<DT>(src, rowbase, col0 + 4 * k, in_len, aligned, pre[rr][it]);
+        }
+    }
+}
+
+template <int DT>
+__device__ __forceinline__ void store_chunk(const uint32_t (&pre)[kPfRows][kPfIters][4], uint2 *buf,
+                                            int K, int cc, int wave, int lane)
+{
+#pragma unroll
+    for (int rr = 0; rr < kPfRows; ++rr) {
+        const int i = wave + kWaves * rr;
+        if (i >= cc) break;
+        uint2 *row = buf + (size_t)i * 4 * K;
+#pragma unroll
+        for (int it = 0; it < kPfIters; ++it) {
+            const int k = lane + 64 * it;
+            if (64 * it >= K) break;
+            if (k < K) store_phases<DT>(row, K, k, pre[rr][it]);
+        }
+    }
+}
+
+template <int DT>
+__device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
+                                 int lane, uint2 *lds)
 {
     const int W = bd.W, K = W >> 2;
     const int64_t rs = bd.src == BUF_IN ? a.in_stride : bd.row_stride;
     const void *src = bd.src == BUF_IN ? a.in : (const void *)a.work;
     const int32_t *relw = a.rel + bd.tab_off + wave * kRPW;
     const int32_t *ab = a.abase + bd.base_off;
+    const int ccr = bd.ccr;
+    const int rb = 32 * K;                     // bytes per staged row
+    const size_t bufsz = (size_t)ccr * 4 * K;  // uint2 per chunk buffer
+    const int nch = (bd.nrows + ccr - 1) / ccr;
+
     float f[kRPW][4];
+    uint32_t ae[kRPW], ao[kRPW];
 #pragma unroll
-    for (int r = 0; r < kRPW; ++r)
+    for (int r = 0; r < kRPW; ++r) {
+        ae[r] = ao[r] = 0u;
 #pragma unroll
         for (int j = 0; j < 4; ++j) f[r][j] = 0.0f;
-
-    for (int c0 = 0; c0 < bd.nrows; c0 += bd.ccr) {
-        const int cc = min(bd.ccr, bd.nrows - c0);
-        __syncthreads();
-        if (bd.dtype == DT_U8)
-            load_tile_int<DT_U8>(lds, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, W, cc,
-                                 bd.in_len, wave, lane);
-        else
-            load_tile_int<DT_U16>(lds, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, W, cc,
-                                  bd.in_len, wave, lane);
-        __syncthreads();
-        uint32_t ae[kRPW], ao[kRPW];
+    }
+    auto flush = [&]() {
 #pragma unroll
-        for (int r = 0; r < kRPW; ++r) ae[r] = ao[r] = 0u;
-        const int rb = 32 * K;  // bytes per staged row
-        const char *lane_base = (const char *)lds + lane * 8;
+        for (int r = 0; r < kRPW; ++r) {
+            f[r][0] += (float)(ae[r] & 0xffffu);
+            f[r][2] += (float)(ae[r] >> 16);
+            f[r][1] += (float)(ao[r] & 0xffffu);
+            f[r][3] += (float)(ao[r] >> 16);
+            ae[r] = ao[r] = 0u;
+        }
+    };
+
+    uint32_t pre[kPfRows][kPfIters][4];
+    __syncthreads();  // the previous item (persistent mode) is done with the buffers
+    fetch_chunk<DT>(pre, src, bd.in_off, bd.row0, rs, ab, t0, K, min(ccr, bd.nrows), bd.in_len,
+                    wave, lane);
+    store_chunk<DT>(pre, lds, K, min(ccr, bd.nrows), wave, lane);
+    __syncthreads();
+    int acc_rows = 0;
+    for (int ch = 0; ch < nch; ++ch) {
+        const int c0 = ch * ccr;
+        const int cc = min(ccr, bd.nrows - c0);
+        const bool more = ch + 1 < nch;
+        const int c1 = c0 + ccr;
+        const int cc1 = more ? min(ccr, bd.nrows - c1) : 0;
+        if (more)  // next chunk's global loads overlap this chunk's compute
+            fetch_chunk<DT>(pre, src, bd.in_off, bd.row0 + c1, rs, ab + c1, t0, K, cc1, bd.in_len,
+                            wave, lane);
+        if (acc_rows + cc > bd.flush) {
+            flush();
+            acc_rows = 0;
+        }
+        const char *lane_base = (const char *)(lds + (ch & 1) * bufsz) + lane * 8;
         int i = 0;
         for (; i + 1 < cc; i += 2) {
             const cint32_t *ri = as_const(relw + (c0 + i) * kDB);
@@ -216,14 +292,11 @@ __device__ void dedisperse_int(const ItemCtx &a, const BlockDesc &bd, int t0, in
                 ao[r] += x.y;
             }
         }
-#pragma unroll
-        for (int r = 0; r < kRPW; ++r) {
-            f[r][0] += (float)(ae[r] & 0xffffu);
-            f[r][2] += (float)(ae[r] >> 16);
-            f[r][1] += (float)(ao[r] & 0xffffu);
-            f[r][3] += (float)(ao[r] >> 16);
-        }
+        acc_rows += cc;
+        if (more) store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, K, cc1, wave, lane);
+        __syncthreads();
     }
+    flush();
 
     float *ob = (bd.dst == BUF_OUT ? a.out : (float *)a.work) + bd.out_off;
     const float kf = (float)bd.k;
@@ -245,6 +318,15 @@ __device__ void dedisperse_int(const ItemCtx &a, const BlockDesc &bd, int t0, in
                 if (t + q < bd.nout) o[t + q] = v[q];
         }
     }
+}
+
+__device__ void dedisperse_int(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
+                               int lane, uint2 *lds)
+{
+    if (bd.dtype == DT_U8)
+        dedisperse_int_t<DT_U8>(a, bd, t0, wave, lane, lds);
+    else
+        dedisperse_int_t<DT_U16>(a, bd, t0, wave, lane, lds);
 }
 
 // ---- float path (float16 / float32 blocks) -------------------------------

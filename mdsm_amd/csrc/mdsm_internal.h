@@ -54,12 +54,15 @@ struct BlockDesc {
     int32_t tab_off;          // int offset of rel[nrows][kDB] in the rel table
     int32_t base_off;         // int offset of abase[nrows] in the base table
     int32_t ccr;              // rows staged per LDS chunk
+    int32_t flush;            // integer path: rows summed in 16-bit lanes between flushes
+    int32_t pad;
 };
 
 // LDS bytes of one staged row: integer data (uint8/uint16) is held as four
 // phase-shifted copies of uint16 pairs (8 bytes per sample), float data as
 // float32 (4 bytes per sample).
 inline int lds_row_bytes(int dtype, int W) { return (dtype == DT_U8 || dtype == DT_U16) ? 8 * W : 4 * W; }
+inline bool int_path(int dtype) { return dtype == DT_U8 || dtype == DT_U16; }
 constexpr int kLdsBudget = 65536;  // bytes of LDS per workgroup
 
 // A run of blocks tiled over the same number of output tiles; flat work

@@ -89,6 +89,7 @@ struct SegSpec {
     int dst;
     int64_t out_off, out_trial_stride;
     int nout, k;
+    int maxval;  // largest sample value of integer data (uint8: 255; k-sums: 255k)
 };
 
 int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
@@ -134,8 +135,16 @@ int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
         b.dtype = s.dtype;
         {
             const int rb = lds_row_bytes(s.dtype, b.W);
-            b.ccr = std::max(1, std::min(kCC, kLdsBudget / rb));
-            st.lds_bytes = std::max(st.lds_bytes, b.ccr * rb);
+            if (int_path(s.dtype)) {
+                // double-buffered chunks of up to 8 rows
+                b.ccr = std::max(2, std::min(8, kLdsBudget / (2 * rb)));
+                st.lds_bytes = std::max(st.lds_bytes, 2 * b.ccr * rb);
+                b.flush = std::max(1, 65535 / std::max(1, s.maxval));
+            } else {
+                b.ccr = std::max(1, std::min(kCC, kLdsBudget / rb));
+                st.lds_bytes = std::max(st.lds_bytes, b.ccr * rb);
+                b.flush = 0;
+            }
         }
         b.tab_off = (int32_t)p->rel.size();
         b.base_off = (int32_t)p->abase.size();
@@ -319,6 +328,7 @@ extern "C" int mdsm_plan_create(const mdsm_obs *obs, const mdsm_range *ranges, u
             s.in_off = 0;
             s.in_len = (int)S;
             s.dtype = obs->dtype;
+            s.maxval = 255;
         } else {
             const DecimStep &d = p->decim[rh.decim];
             s.src = BUF_WORK;
@@ -326,6 +336,7 @@ extern "C" int mdsm_plan_create(const mdsm_obs *obs, const mdsm_range *ranges, u
             s.in_off = d.work_off / (int64_t)dtype_size(d.dtype_out);
             s.in_len = d.nsamp_out;
             s.dtype = d.dtype_out;
+            s.maxval = 255 * d.k;
         }
         s.row0 = 0;
         s.dst = BUF_OUT;
@@ -478,6 +489,7 @@ extern "C" int mdsm_plan_create_subband(const mdsm_obs *obs, uint32_t nsubs, con
                 s.in_off = 0;
                 s.in_len = (int)S;
                 s.dtype = obs->dtype;
+                s.maxval = 255;
             } else {
                 const DecimStep &d = p->decim[rh.decim];
                 s.src = BUF_WORK;
@@ -485,6 +497,7 @@ extern "C" int mdsm_plan_create_subband(const mdsm_obs *obs, uint32_t nsubs, con
                 s.in_off = d.work_off / (int64_t)dtype_size(d.dtype_out);
                 s.in_len = d.nsamp_out;
                 s.dtype = d.dtype_out;
+                s.maxval = 255 * d.k;
             }
             s.row0 = top;
             s.dst = BUF_WORK;
@@ -517,6 +530,7 @@ extern "C" int mdsm_plan_create_subband(const mdsm_obs *obs, uint32_t nsubs, con
             s.in_off = rh.sb_off / 4 + (int64_t)j * nsubs * rh.len1;
             s.in_len = rh.len1;
             s.dtype = DT_F32;
+            s.maxval = 0;
             s.dst = BUF_OUT;
             s.out_off = (int64_t)rh.info.out_offset + (int64_t)d0 * rh.info.nout;
             s.out_trial_stride = rh.info.nout;
