@@ -21,7 +21,7 @@ from typing import Sequence
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libmdsm_hip.so")
+LIB_PATH = os.environ.get("MDSM_LIB", os.path.join(HERE, "libmdsm_hip.so"))
 
 U8, F16, F32 = 0, 1, 2
 SHIFT_REF, SHIFT_NS = 0, 1

@@ -14,8 +14,8 @@ import sys
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
-BUILD = os.path.join(HERE, "_build")
-LIB = os.path.join(HERE, "libmdsm_hip.so")
+BUILD = os.environ.get("MDSM_BUILD_DIR", os.path.join(HERE, "_build"))
+LIB = os.environ.get("MDSM_BUILD_LIB", os.path.join(HERE, "libmdsm_hip.so"))
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 COMMON = [
@@ -24,7 +24,7 @@ COMMON = [
     "--offload-arch=gfx950",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
     "-I" + os.path.join(HERE, "..", "include"),
-]
+] + os.environ.get("MDSM_EXTRA_FLAGS", "").split()
 SOURCES = sorted(f for f in os.listdir(CSRC) if f.endswith((".hip", ".cpp")))
 
 

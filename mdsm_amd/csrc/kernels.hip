@@ -39,6 +39,30 @@ __device__ __forceinline__ const cint32_t *as_const(const int32_t *p)
     return (const cint32_t *)(p);
 }
 
+// Constant-address-space view of a read-only table: uniform loads through it
+// are scalar (s_load); loads through generic pointers never are.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MDSM_CONST_AS __attribute__((address_space(4)))
+#else
+#define MDSM_CONST_AS  // the host pass never runs device code
+#endif
+template <typename T>
+__device__ __forceinline__ const MDSM_CONST_AS T *cptr(const T *p)
+{
+    return (const MDSM_CONST_AS T *)(p);
+}
+
+// A pointer every lane holds the same value of, moved to scalar registers so
+// that loads through it compile to s_load.
+template <typename T>
+__device__ __forceinline__ T *uniform_ptr(T *p)
+{
+    const uint64_t v = (uint64_t)p;
+    const uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)v);
+    const uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)(v >> 32));
+    return (T *)(((uint64_t)hi << 32) | lo);
+}
+
 struct DedispArgs {
     const SegDesc *segs;
     int64_t items;
@@ -214,19 +238,39 @@ __device__ __forceinline__ void store_chunk(const uint32_t (&pre)[kPfRows][kPfIt
     }
 }
 
+// Build-time variants of the integer path (timed against each other):
+//   MDSM_IV_OFFS  1: per-chunk offsets staged in LDS with the data; 0: scalar loads
+//   MDSM_IV_PF    1: next chunk prefetched into registers (double-buffered LDS);
+//                 0: load, barrier, sum, barrier (single buffer)
+//   MDSM_IV_SB    1: all 16 reads of a channel pair issued before its adds
+#ifndef MDSM_IV_OFFS
+#define MDSM_IV_OFFS 0
+#endif
+#ifndef MDSM_IV_PF
+#define MDSM_IV_PF 1
+#endif
+#ifndef MDSM_IV_SB
+#define MDSM_IV_SB 1
+#endif
+
 template <int DT>
 __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
                                  int lane, uint2 *lds)
 {
+    constexpr bool kOffLds = MDSM_IV_OFFS != 0;
+    constexpr bool kPf = MDSM_IV_PF != 0;
     const int W = bd.W, K = W >> 2;
     const int64_t rs = bd.src == BUF_IN ? a.in_stride : bd.row_stride;
     const void *src = bd.src == BUF_IN ? a.in : (const void *)a.work;
-    const int32_t *relw = a.rel + bd.tab_off + wave * kRPW;
-    const int32_t *ab = a.abase + bd.base_off;
+    const int32_t *relb = uniform_ptr(a.rel + bd.tab_off);  // [nrows][kDB] (+8 zero rows)
+    const int32_t *ab = uniform_ptr(a.abase + bd.base_off);
     const int ccr = bd.ccr;
     const int rb = 32 * K;                     // bytes per staged row
     const size_t bufsz = (size_t)ccr * 4 * K;  // uint2 per chunk buffer
     const int nch = (bd.nrows + ccr - 1) / ccr;
+    // per-chunk offset tables follow the two data buffers: [2][8 rows][kDB]
+    int32_t *offs = (int32_t *)(lds + 2 * bufsz);
+    const int tid = threadIdx.x;
 
     float f[kRPW][4];
     uint32_t ae[kRPW], ao[kRPW];
@@ -246,55 +290,129 @@ __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, 
             ae[r] = ao[r] = 0u;
         }
     };
-
-    uint32_t pre[kPfRows][kPfIters][4];
-    __syncthreads();  // the previous item (persistent mode) is done with the buffers
-    fetch_chunk<DT>(pre, src, bd.in_off, bd.row0, rs, ab, t0, K, min(ccr, bd.nrows), bd.in_len,
-                    wave, lane);
-    store_chunk<DT>(pre, lds, K, min(ccr, bd.nrows), wave, lane);
-    __syncthreads();
-    int acc_rows = 0;
-    for (int ch = 0; ch < nch; ++ch) {
-        const int c0 = ch * ccr;
-        const int cc = min(ccr, bd.nrows - c0);
-        const bool more = ch + 1 < nch;
-        const int c1 = c0 + ccr;
-        const int cc1 = more ? min(ccr, bd.nrows - c1) : 0;
-        if (more)  // next chunk's global loads overlap this chunk's compute
-            fetch_chunk<DT>(pre, src, bd.in_off, bd.row0 + c1, rs, ab + c1, t0, K, cc1, bd.in_len,
-                            wave, lane);
-        if (acc_rows + cc > bd.flush) {
-            flush();
-            acc_rows = 0;
-        }
-        const char *lane_base = (const char *)(lds + (ch & 1) * bufsz) + lane * 8;
+    // sum rows [0, cc) of the staged chunk into the packed accumulators
+    auto sum_chunk = [&](const char *lane_base, const int32_t *orow, const int32_t *grow,
+                         int cc) {
         int i = 0;
         for (; i + 1 < cc; i += 2) {
-            const cint32_t *ri = as_const(relw + (c0 + i) * kDB);
-            const cint32_t *rj = ri + kDB;
+            int oi[kRPW], oj[kRPW];
+            if constexpr (kOffLds) {
+                const int4 a0 = *(const int4 *)(orow + i * kDB);
+                const int4 a1 = *(const int4 *)(orow + i * kDB + 4);
+                const int4 b0 = *(const int4 *)(orow + (i + 1) * kDB);
+                const int4 b1 = *(const int4 *)(orow + (i + 1) * kDB + 4);
+                oi[0] = a0.x; oi[1] = a0.y; oi[2] = a0.z; oi[3] = a0.w;
+                oi[4] = a1.x; oi[5] = a1.y; oi[6] = a1.z; oi[7] = a1.w;
+                oj[0] = b0.x; oj[1] = b0.y; oj[2] = b0.z; oj[3] = b0.w;
+                oj[4] = b1.x; oj[5] = b1.y; oj[6] = b1.z; oj[7] = b1.w;
+            } else {
+                const cint32_t *ri = as_const(grow + i * kDB);
+#pragma unroll
+                for (int r = 0; r < kRPW; ++r) {
+                    oi[r] = ri[r];
+                    oj[r] = ri[kDB + r];
+                }
+            }
             const char *rowi = lane_base + i * rb;
             const char *rowj = rowi + rb;
+            uint2 x[kRPW], y[kRPW];
 #pragma unroll
             for (int r = 0; r < kRPW; ++r) {
-                const uint2 x = *(const uint2 *)(rowi + ri[r]);
-                const uint2 y = *(const uint2 *)(rowj + rj[r]);
-                ae[r] = ae[r] + x.x + y.x;
-                ao[r] = ao[r] + x.y + y.y;
+                x[r] = *(const uint2 *)(rowi + oi[r]);
+                y[r] = *(const uint2 *)(rowj + oj[r]);
+            }
+#if MDSM_IV_SB
+            __builtin_amdgcn_sched_barrier(0);
+#endif
+#pragma unroll
+            for (int r = 0; r < kRPW; ++r) {
+                ae[r] = ae[r] + x[r].x + y[r].x;
+                ao[r] = ao[r] + x[r].y + y[r].y;
             }
         }
         if (i < cc) {
-            const cint32_t *ri = as_const(relw + (c0 + i) * kDB);
+            int oi[kRPW];
+            if constexpr (kOffLds) {
+                const int4 a0 = *(const int4 *)(orow + i * kDB);
+                const int4 a1 = *(const int4 *)(orow + i * kDB + 4);
+                oi[0] = a0.x; oi[1] = a0.y; oi[2] = a0.z; oi[3] = a0.w;
+                oi[4] = a1.x; oi[5] = a1.y; oi[6] = a1.z; oi[7] = a1.w;
+            } else {
+                const cint32_t *ri = as_const(grow + i * kDB);
+#pragma unroll
+                for (int r = 0; r < kRPW; ++r) oi[r] = ri[r];
+            }
             const char *rowi = lane_base + i * rb;
 #pragma unroll
             for (int r = 0; r < kRPW; ++r) {
-                const uint2 x = *(const uint2 *)(rowi + ri[r]);
+                const uint2 x = *(const uint2 *)(rowi + oi[r]);
                 ae[r] += x.x;
                 ao[r] += x.y;
             }
         }
-        acc_rows += cc;
-        if (more) store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, K, cc1, wave, lane);
+    };
+
+    int acc_rows = 0;
+    if constexpr (kPf) {
+        uint32_t pre[kPfRows][kPfIters][4];
+        __syncthreads();  // the previous item (persistent mode) is done with the buffers
+        {
+            const int cc0 = min(ccr, bd.nrows);
+            fetch_chunk<DT>(pre, src, bd.in_off, bd.row0, rs, ab, t0, K, cc0, bd.in_len, wave,
+                            lane);
+            int32_t ov = 0;
+            if constexpr (kOffLds) ov = relb[tid];
+            store_chunk<DT>(pre, lds, K, cc0, wave, lane);
+            if constexpr (kOffLds) offs[tid] = ov;
+        }
         __syncthreads();
+        for (int ch = 0; ch < nch; ++ch) {
+            const int c0 = ch * ccr;
+            const int cc = min(ccr, bd.nrows - c0);
+            const bool more = ch + 1 < nch;
+            const int c1 = c0 + ccr;
+            const int cc1 = more ? min(ccr, bd.nrows - c1) : 0;
+            int32_t ov1 = 0;
+            if (more) {  // next chunk's global loads overlap this chunk's compute
+                fetch_chunk<DT>(pre, src, bd.in_off, bd.row0 + c1, rs, ab + c1, t0, K, cc1,
+                                bd.in_len, wave, lane);
+                if constexpr (kOffLds) ov1 = relb[(size_t)c1 * kDB + tid];
+            }
+            if (acc_rows + cc > bd.flush) {
+                flush();
+                acc_rows = 0;
+            }
+            sum_chunk((const char *)(lds + (ch & 1) * bufsz) + lane * 8,
+                      offs + (ch & 1) * (8 * kDB) + wave * kRPW,
+                      relb + (size_t)c0 * kDB + wave * kRPW, cc);
+            acc_rows += cc;
+            if (more) {
+                store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, K, cc1, wave, lane);
+                if constexpr (kOffLds) offs[((ch + 1) & 1) * (8 * kDB) + tid] = ov1;
+            }
+            __syncthreads();
+        }
+    } else {
+        for (int ch = 0; ch < nch; ++ch) {
+            const int c0 = ch * ccr;
+            const int cc = min(ccr, bd.nrows - c0);
+            uint32_t pre[kPfRows][kPfIters][4];
+            __syncthreads();
+            fetch_chunk<DT>(pre, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, K, cc, bd.in_len,
+                            wave, lane);
+            int32_t ov = 0;
+            if constexpr (kOffLds) ov = relb[(size_t)c0 * kDB + tid];
+            store_chunk<DT>(pre, lds, K, cc, wave, lane);
+            if constexpr (kOffLds) offs[tid] = ov;
+            __syncthreads();
+            if (acc_rows + cc > bd.flush) {
+                flush();
+                acc_rows = 0;
+            }
+            sum_chunk((const char *)lds + lane * 8, offs + wave * kRPW,
+                      relb + (size_t)c0 * kDB + wave * kRPW, cc);
+            acc_rows += cc;
+        }
     }
     flush();
 
@@ -336,8 +454,8 @@ __device__ void dedisperse_float(const ItemCtx &a, const BlockDesc &bd, int t0, 
     const int W = bd.W;
     const int64_t rs = bd.src == BUF_IN ? a.in_stride : bd.row_stride;
     const void *src = bd.src == BUF_IN ? a.in : (const void *)a.work;
-    const int32_t *relw = a.rel + bd.tab_off + wave * kRPW;
-    const int32_t *ab = a.abase + bd.base_off;
+    const int32_t *relw = uniform_ptr(a.rel + bd.tab_off + wave * kRPW);
+    const int32_t *ab = uniform_ptr(a.abase + bd.base_off);
 
     float acc[kRPW][kPPL];
 #pragma unroll
@@ -390,18 +508,21 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
     int lo = a.seg_begin, hi = a.seg_begin + a.nseg - 1;
     while (lo < hi) {
         const int mid = (lo + hi + 1) >> 1;
-        if (a.segs[mid].item_start <= f)
+        if (cptr(a.segs)[mid].item_start <= f)
             lo = mid;
         else
             hi = mid - 1;
     }
-    const SegDesc sg = a.segs[lo];
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    const SegDesc sg = *cptr(uniform_ptr(a.segs + lo));
     const int64_t fl = f - sg.item_start;
-    const int tile_i = (int)(fl / sg.nblocks);
-    const int bi = sg.first_block + (int)(fl - (int64_t)tile_i * sg.nblocks);
-    const BlockDesc bd = sg.blocks[bi];
+    const int tile_i = __builtin_amdgcn_readfirstlane((int)(fl / sg.nblocks));
+    const int bi = __builtin_amdgcn_readfirstlane(sg.first_block +
+                                                  (int)(fl - (int64_t)tile_i * sg.nblocks));
+    const BlockDesc bd = *cptr(uniform_ptr(sg.blocks + bi));
     const JobBufs jb = a.jobs[sg.job];
-    ItemCtx ctx{sg.rel, sg.abase, jb.in, jb.out, jb.work, jb.in_stride};
+    ItemCtx ctx{uniform_ptr(sg.rel), uniform_ptr(sg.abase), uniform_ptr(jb.in),
+                uniform_ptr(jb.out), uniform_ptr(jb.work), jb.in_stride};
     const int t0 = tile_i * kTile;
     if (bd.dtype == DT_U8 || bd.dtype == DT_U16)
         dedisperse_int(ctx, bd, t0, wave, lane, lds);
@@ -431,7 +552,12 @@ __global__ void __launch_bounds__(256) dedisperse_kernel(DedispArgs a)
         __syncthreads();
         if (threadIdx.x == 0) sh_item = (int64_t)atomicAdd(a.counter, 1u);
         __syncthreads();
-        const int64_t f = sh_item;
+        const int64_t fv = sh_item;
+        // the item index is the same in every lane: make that explicit so the
+        // descriptor and table loads that depend on it stay scalar
+        const uint32_t flo = __builtin_amdgcn_readfirstlane((uint32_t)fv);
+        const uint32_t fhi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)fv >> 32));
+        const int64_t f = (int64_t)(((uint64_t)fhi << 32) | flo);
         if (f >= a.items) break;
         dedisperse_item(a, f, wave, lane, lds_raw);
     }

@@ -136,9 +136,9 @@ int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
         {
             const int rb = lds_row_bytes(s.dtype, b.W);
             if (int_path(s.dtype)) {
-                // double-buffered chunks of up to 8 rows
-                b.ccr = std::max(2, std::min(8, kLdsBudget / (2 * rb)));
-                st.lds_bytes = std::max(st.lds_bytes, 2 * b.ccr * rb);
+                // double-buffered chunks of up to 8 rows, plus two 8-row offset tables
+                b.ccr = std::max(2, std::min(8, (kLdsBudget - 2048) / (2 * rb)));
+                st.lds_bytes = std::max(st.lds_bytes, 2 * b.ccr * rb + 2 * 8 * kDB * 4);
                 b.flush = std::max(1, 65535 / std::max(1, s.maxval));
             } else {
                 b.ccr = std::max(1, std::min(kCC, kLdsBudget / rb));
@@ -165,6 +165,9 @@ int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
         p->blocks.push_back(b);
         j0 += nt;
     }
+    // 8 zero rows after every block's table: the integer path stages 8 rows of
+    // offsets per chunk without a bounds test
+    for (int i = 0; i < 8 * kDB; ++i) p->rel.push_back(0);
     seg.nblocks = (int32_t)p->blocks.size() - seg.first_block;
     seg.item_start = st.items;
     st.items += (int64_t)seg.nblocks * seg.ntiles;
