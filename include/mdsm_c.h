@@ -146,6 +146,32 @@ int mdsm_plan_fine_table(const mdsm_plan *plan, uint32_t r, int32_t *host_out);
  * (src/pelican-mdsm/modules/MdsmModule.cpp:189,202-210). */
 int mdsm_plan_dm_values(const mdsm_plan *plan, float *host_out);
 
+/* Launch shape of a plan's dedispersion kernels: the autotuner's search
+ * space.  Results do not depend on it (integer sums are exact, float sums
+ * keep their channel order).  No counterpart in the reference, whose launch
+ * shapes are compile-time constants (src/mdsm/cache_brute_force.h:4-6,
+ * src/mdsm/dedispersion_thread.cu:69-79). */
+typedef struct {
+    uint32_t chunk_rows;  /* channels staged per LDS chunk: integer path 2..8,
+                             float path 1..16; 0 = as many as lds_budget holds */
+    uint32_t lds_budget;  /* LDS bytes per workgroup for staged channels;
+                             0 = 65536 (the most allowed) */
+    uint32_t persistent;  /* 1: persistent workgroups over a device work queue */
+    uint32_t grid;        /* persistent workgroups; 0 = 8 per compute unit */
+} mdsm_tuning;
+
+int mdsm_plan_set_tuning(mdsm_plan *plan, const mdsm_tuning *t);
+int mdsm_plan_get_tuning(const mdsm_plan *plan, mdsm_tuning *t);
+
+/* Time the candidate launch shapes on the caller's block (reps timed runs
+ * each, after one warm-up), keep the fastest, and remember it for every
+ * later plan of the same shape in this process.  d_out/d_work as for
+ * mdsm_dedisperse; the block is dedispersed several times.  best may be
+ * NULL. */
+int mdsm_plan_autotune(mdsm_plan *plan, const void *d_in, int64_t in_stride,
+                       float *d_out, void *d_work, void *stream, uint32_t reps,
+                       mdsm_tuning *best);
+
 /* ------------------------------------------------------------------ */
 /* Dedispersion                                                        */
 /* ------------------------------------------------------------------ */

@@ -63,6 +63,14 @@ class PlanInfo(C.Structure):
                 ("is_subband", C.c_uint32), ("nblocks", C.c_uint32)]
 
 
+class Tuning(C.Structure):
+    _fields_ = [("chunk_rows", C.c_uint32), ("lds_budget", C.c_uint32),
+                ("persistent", C.c_uint32), ("grid", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {f: int(getattr(self, f)) for f, _ in self._fields_}
+
+
 class RangeInfo(C.Structure):
     _fields_ = [("lowdm", C.c_float), ("dmstep", C.c_float), ("ndms", C.c_uint32),
                 ("downsample", C.c_uint32), ("nout", C.c_uint32), ("maxshift", C.c_uint32),
@@ -131,6 +139,9 @@ def lib():
         "mdsm_timing_enable": [C.c_int],
         "mdsm_timing_reset": [],
         "mdsm_timing_get": [C.c_char_p, C.POINTER(f64), C.POINTER(u64)],
+        "mdsm_plan_set_tuning": [vp, C.POINTER(Tuning)],
+        "mdsm_plan_get_tuning": [vp, C.POINTER(Tuning)],
+        "mdsm_plan_autotune": [vp, vp, i64, vp, vp, vp, u32, C.POINTER(Tuning)],
         "mdsm_device_count": [C.POINTER(C.c_int)],
         "mdsm_device_set": [C.c_int],
         "mdsm_device_query": [C.c_int, C.POINTER(DeviceProps)],
@@ -360,6 +371,24 @@ class Plan:
         pout = d_out.ptr if isinstance(d_out, DeviceBuffer) else d_out
         pw = d_work.ptr if isinstance(d_work, DeviceBuffer) else d_work
         check(lib().mdsm_dedisperse(self._h, pin, int(in_stride), pout, pw, _sh(stream)))
+
+    def set_tuning(self, chunk_rows=0, lds_budget=0, persistent=0, grid=0):
+        t = Tuning(chunk_rows, lds_budget, persistent, grid)
+        check(lib().mdsm_plan_set_tuning(self._h, C.byref(t)))
+
+    def tuning(self) -> dict:
+        t = Tuning()
+        check(lib().mdsm_plan_get_tuning(self._h, C.byref(t)))
+        return t.as_dict()
+
+    def autotune(self, d_in, in_stride: int, d_out, d_work=None, stream=None, reps: int = 3) -> dict:
+        pin = d_in.ptr if isinstance(d_in, DeviceBuffer) else d_in
+        pout = d_out.ptr if isinstance(d_out, DeviceBuffer) else d_out
+        pw = d_work.ptr if isinstance(d_work, DeviceBuffer) else d_work
+        t = Tuning()
+        check(lib().mdsm_plan_autotune(self._h, pin, int(in_stride), pout, pw, _sh(stream),
+                                       int(reps), C.byref(t)))
+        return t.as_dict()
 
     def split_output(self, flat: np.ndarray) -> list:
         return [flat[R.out_offset:R.out_offset + R.ndms * R.nout].reshape(R.ndms, R.nout)

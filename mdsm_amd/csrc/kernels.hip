@@ -1,13 +1,16 @@
 // gfx950 kernels of the dedispersion path.
 //
-// dedisperse_kernel: one workgroup (4 waves) per work item = (block of
-//   kDB=32 DM trials sharing one set of input rows, tile of 256 output
-//   samples).  Channels are staged 16 at a time into LDS as float32 rows
-//   covering [t0 + abase_c, t0 + abase_c + W); each wave holds 8 trials x 4
-//   samples of accumulators in registers and reads its per-(row, trial)
-//   offsets as wave-uniform scalars.  Work items are ordered tile-major and
-//   remapped so that each XCD (8 per MI355X, round-robin dispatch) receives a
-//   contiguous run: blocks sharing an input tile meet in the same L2.
+// dedisperse_kernel: one workgroup (8 waves) per work item = (block of
+//   kDB=64 DM trials sharing one set of input rows, tile of 256 output
+//   samples).  Each wave holds 8 trials x 4 samples of accumulators in
+//   registers and reads its per-(channel, trial) offsets as wave-uniform
+//   scalars.  Integer data (uint8, and the uint16 sums of decimated ranges)
+//   is staged a chunk of channels at a time into double-buffered LDS in a
+//   four-phase uint16 layout, so that any offset is one aligned ds_read_b64
+//   per lane; float data is staged as float32 rows.  Work items are ordered
+//   tile-major and remapped so that each XCD (8 per MI355X, round-robin
+//   dispatch) receives a contiguous run: blocks sharing an input tile meet in
+//   the same L2.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -193,15 +196,17 @@ __device__ __forceinline__ void store_phases(uint2 *row, int K, int k, const uin
     for (int ph = 0; ph < 4; ++ph) row[ph * K + k] = make_uint2(d0[ph], d1[ph]);
 }
 
-// Register prefetch of one chunk: wave w owns rows w and w+4, each up to 4
-// pair columns per lane (K <= 256).
-constexpr int kPfRows = 2, kPfIters = 4;
+// Register prefetch of one chunk: wave w owns rows w, w+kWaves, ... of the
+// chunk (<= 8 rows), each up to 4 pair columns per lane (K <= 256).
+constexpr int kPfRows = (8 + kWaves - 1) / kWaves, kPfIters = 4;
 
 template <int DT>
-__device__ __forceinline__ void fetch_chunk(uint32_t (&pre)[kPfRows][kPfIters][4], const void *src,
-                                            int64_t in_off, int64_t row0, int64_t rs,
-                                            const int32_t *ab, int t0, int K, int cc, int in_len,
-                                            int wave, int lane)
+using PreRegs = uint32_t[kPfRows][kPfIters][RawWords<DT>::n];
+
+template <int DT>
+__device__ __forceinline__ void fetch_chunk(PreRegs<DT> &pre, const void *src, int64_t in_off,
+                                            int64_t row0, int64_t rs, const int32_t *ab, int t0,
+                                            int K, int cc, int in_len, int wave, int lane)
 {
     constexpr int es = DT == DT_U8 ? 1 : 2;
 #pragma unroll
@@ -221,8 +226,8 @@ __device__ __forceinline__ void fetch_chunk(uint32_t (&pre)[kPfRows][kPfIters][4
 }
 
 template <int DT>
-__device__ __forceinline__ void store_chunk(const uint32_t (&pre)[kPfRows][kPfIters][4], uint2 *buf,
-                                            int K, int cc, int wave, int lane)
+__device__ __forceinline__ void store_chunk(const PreRegs<DT> &pre, uint2 *buf, int K, int cc,
+                                            int wave, int lane)
 {
 #pragma unroll
     for (int rr = 0; rr < kPfRows; ++rr) {
@@ -238,27 +243,14 @@ __device__ __forceinline__ void store_chunk(const uint32_t (&pre)[kPfRows][kPfIt
     }
 }
 
-// Build-time variants of the integer path (timed against each other):
-//   MDSM_IV_OFFS  1: per-chunk offsets staged in LDS with the data; 0: scalar loads
-//   MDSM_IV_PF    1: next chunk prefetched into registers (double-buffered LDS);
-//                 0: load, barrier, sum, barrier (single buffer)
-//   MDSM_IV_SB    1: all 16 reads of a channel pair issued before its adds
-#ifndef MDSM_IV_OFFS
-#define MDSM_IV_OFFS 0
-#endif
-#ifndef MDSM_IV_PF
-#define MDSM_IV_PF 1
-#endif
-#ifndef MDSM_IV_SB
-#define MDSM_IV_SB 1
-#endif
-
+// Integer path: chunks of ccr <= 8 channels are double-buffered in LDS; the
+// next chunk's global loads are in flight (in registers) while the current
+// chunk is summed.  Each wave reads its 8 trials' offsets of a channel pair
+// with two scalar loads and issues 16 ds_read_b64 before the adds.
 template <int DT>
 __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
                                  int lane, uint2 *lds)
 {
-    constexpr bool kOffLds = MDSM_IV_OFFS != 0;
-    constexpr bool kPf = MDSM_IV_PF != 0;
     const int W = bd.W, K = W >> 2;
     const int64_t rs = bd.src == BUF_IN ? a.in_stride : bd.row_stride;
     const void *src = bd.src == BUF_IN ? a.in : (const void *)a.work;
@@ -268,9 +260,6 @@ __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, 
     const int rb = 32 * K;                     // bytes per staged row
     const size_t bufsz = (size_t)ccr * 4 * K;  // uint2 per chunk buffer
     const int nch = (bd.nrows + ccr - 1) / ccr;
-    // per-chunk offset tables follow the two data buffers: [2][8 rows][kDB]
-    int32_t *offs = (int32_t *)(lds + 2 * bufsz);
-    const int tid = threadIdx.x;
 
     float f[kRPW][4];
     uint32_t ae[kRPW], ao[kRPW];
@@ -291,27 +280,15 @@ __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, 
         }
     };
     // sum rows [0, cc) of the staged chunk into the packed accumulators
-    auto sum_chunk = [&](const char *lane_base, const int32_t *orow, const int32_t *grow,
-                         int cc) {
+    auto sum_chunk = [&](const char *lane_base, const int32_t *grow, int cc) {
         int i = 0;
         for (; i + 1 < cc; i += 2) {
             int oi[kRPW], oj[kRPW];
-            if constexpr (kOffLds) {
-                const int4 a0 = *(const int4 *)(orow + i * kDB);
-                const int4 a1 = *(const int4 *)(orow + i * kDB + 4);
-                const int4 b0 = *(const int4 *)(orow + (i + 1) * kDB);
-                const int4 b1 = *(const int4 *)(orow + (i + 1) * kDB + 4);
-                oi[0] = a0.x; oi[1] = a0.y; oi[2] = a0.z; oi[3] = a0.w;
-                oi[4] = a1.x; oi[5] = a1.y; oi[6] = a1.z; oi[7] = a1.w;
-                oj[0] = b0.x; oj[1] = b0.y; oj[2] = b0.z; oj[3] = b0.w;
-                oj[4] = b1.x; oj[5] = b1.y; oj[6] = b1.z; oj[7] = b1.w;
-            } else {
-                const cint32_t *ri = as_const(grow + i * kDB);
+            const cint32_t *ri = as_const(grow + i * kDB);
 #pragma unroll
-                for (int r = 0; r < kRPW; ++r) {
-                    oi[r] = ri[r];
-                    oj[r] = ri[kDB + r];
-                }
+            for (int r = 0; r < kRPW; ++r) {
+                oi[r] = ri[r];
+                oj[r] = ri[kDB + r];
             }
             const char *rowi = lane_base + i * rb;
             const char *rowj = rowi + rb;
@@ -321,9 +298,6 @@ __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, 
                 x[r] = *(const uint2 *)(rowi + oi[r]);
                 y[r] = *(const uint2 *)(rowj + oj[r]);
             }
-#if MDSM_IV_SB
-            __builtin_amdgcn_sched_barrier(0);
-#endif
 #pragma unroll
             for (int r = 0; r < kRPW; ++r) {
                 ae[r] = ae[r] + x[r].x + y[r].x;
@@ -332,16 +306,9 @@ __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, 
         }
         if (i < cc) {
             int oi[kRPW];
-            if constexpr (kOffLds) {
-                const int4 a0 = *(const int4 *)(orow + i * kDB);
-                const int4 a1 = *(const int4 *)(orow + i * kDB + 4);
-                oi[0] = a0.x; oi[1] = a0.y; oi[2] = a0.z; oi[3] = a0.w;
-                oi[4] = a1.x; oi[5] = a1.y; oi[6] = a1.z; oi[7] = a1.w;
-            } else {
-                const cint32_t *ri = as_const(grow + i * kDB);
+            const cint32_t *ri = as_const(grow + i * kDB);
 #pragma unroll
-                for (int r = 0; r < kRPW; ++r) oi[r] = ri[r];
-            }
+            for (int r = 0; r < kRPW; ++r) oi[r] = ri[r];
             const char *rowi = lane_base + i * rb;
 #pragma unroll
             for (int r = 0; r < kRPW; ++r) {
@@ -353,66 +320,32 @@ __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, 
     };
 
     int acc_rows = 0;
-    if constexpr (kPf) {
-        uint32_t pre[kPfRows][kPfIters][4];
-        __syncthreads();  // the previous item (persistent mode) is done with the buffers
-        {
-            const int cc0 = min(ccr, bd.nrows);
-            fetch_chunk<DT>(pre, src, bd.in_off, bd.row0, rs, ab, t0, K, cc0, bd.in_len, wave,
-                            lane);
-            int32_t ov = 0;
-            if constexpr (kOffLds) ov = relb[tid];
-            store_chunk<DT>(pre, lds, K, cc0, wave, lane);
-            if constexpr (kOffLds) offs[tid] = ov;
-        }
-        __syncthreads();
-        for (int ch = 0; ch < nch; ++ch) {
-            const int c0 = ch * ccr;
-            const int cc = min(ccr, bd.nrows - c0);
-            const bool more = ch + 1 < nch;
-            const int c1 = c0 + ccr;
-            const int cc1 = more ? min(ccr, bd.nrows - c1) : 0;
-            int32_t ov1 = 0;
-            if (more) {  // next chunk's global loads overlap this chunk's compute
-                fetch_chunk<DT>(pre, src, bd.in_off, bd.row0 + c1, rs, ab + c1, t0, K, cc1,
-                                bd.in_len, wave, lane);
-                if constexpr (kOffLds) ov1 = relb[(size_t)c1 * kDB + tid];
-            }
-            if (acc_rows + cc > bd.flush) {
-                flush();
-                acc_rows = 0;
-            }
-            sum_chunk((const char *)(lds + (ch & 1) * bufsz) + lane * 8,
-                      offs + (ch & 1) * (8 * kDB) + wave * kRPW,
-                      relb + (size_t)c0 * kDB + wave * kRPW, cc);
-            acc_rows += cc;
-            if (more) {
-                store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, K, cc1, wave, lane);
-                if constexpr (kOffLds) offs[((ch + 1) & 1) * (8 * kDB) + tid] = ov1;
-            }
-            __syncthreads();
-        }
-    } else {
-        for (int ch = 0; ch < nch; ++ch) {
-            const int c0 = ch * ccr;
-            const int cc = min(ccr, bd.nrows - c0);
-            uint32_t pre[kPfRows][kPfIters][4];
-            __syncthreads();
-            fetch_chunk<DT>(pre, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, K, cc, bd.in_len,
+    PreRegs<DT> pre;
+    __syncthreads();  // the previous item (persistent mode) is done with the buffers
+    {
+        const int cc0 = min(ccr, bd.nrows);
+        fetch_chunk<DT>(pre, src, bd.in_off, bd.row0, rs, ab, t0, K, cc0, bd.in_len, wave, lane);
+        store_chunk<DT>(pre, lds, K, cc0, wave, lane);
+    }
+    __syncthreads();
+    for (int ch = 0; ch < nch; ++ch) {
+        const int c0 = ch * ccr;
+        const int cc = min(ccr, bd.nrows - c0);
+        const bool more = ch + 1 < nch;
+        const int c1 = c0 + ccr;
+        const int cc1 = more ? min(ccr, bd.nrows - c1) : 0;
+        if (more)  // next chunk's global loads overlap this chunk's compute
+            fetch_chunk<DT>(pre, src, bd.in_off, bd.row0 + c1, rs, ab + c1, t0, K, cc1, bd.in_len,
                             wave, lane);
-            int32_t ov = 0;
-            if constexpr (kOffLds) ov = relb[(size_t)c0 * kDB + tid];
-            store_chunk<DT>(pre, lds, K, cc, wave, lane);
-            if constexpr (kOffLds) offs[tid] = ov;
-            __syncthreads();
-            if (acc_rows + cc > bd.flush) {
-                flush();
-                acc_rows = 0;
-            }
-            sum_chunk((const char *)lds + lane * 8, offs + wave * kRPW,
-                      relb + (size_t)c0 * kDB + wave * kRPW, cc);
-            acc_rows += cc;
+        if (acc_rows + cc > bd.flush) {
+            flush();
+            acc_rows = 0;
         }
+        sum_chunk((const char *)(lds + (ch & 1) * bufsz) + lane * 8,
+                  relb + (size_t)c0 * kDB + wave * kRPW, cc);
+        acc_rows += cc;
+        if (more) store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, K, cc1, wave, lane);
+        __syncthreads();
     }
     flush();
 
@@ -524,13 +457,23 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
     ItemCtx ctx{uniform_ptr(sg.rel), uniform_ptr(sg.abase), uniform_ptr(jb.in),
                 uniform_ptr(jb.out), uniform_ptr(jb.work), jb.in_stride};
     const int t0 = tile_i * kTile;
+#if defined(MDSM_ONLY_INT)
+    dedisperse_int(ctx, bd, t0, wave, lane, lds);
+#elif defined(MDSM_ONLY_FLOAT)
+    dedisperse_float(ctx, bd, t0, wave, lane, (float *)lds);
+#else
     if (bd.dtype == DT_U8 || bd.dtype == DT_U16)
         dedisperse_int(ctx, bd, t0, wave, lane, lds);
     else
         dedisperse_float(ctx, bd, t0, wave, lane, (float *)lds);
+#endif
 }
 
-__global__ void __launch_bounds__(256) dedisperse_kernel(DedispArgs a)
+#ifndef MDSM_DEDISP_WAVES
+#define MDSM_DEDISP_WAVES 4
+#endif
+__global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MDSM_DEDISP_WAVES)))
+dedisperse_kernel(DedispArgs a)
 {
     extern __shared__ uint2 lds_raw[];
     __shared__ int64_t sh_item;
@@ -580,7 +523,7 @@ hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s)
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, L.counter, 1u, 0u);
         grid = (unsigned)std::min<int64_t>(L.grid > 0 ? L.grid : 2048, L.items);
     }
-    hipLaunchKernelGGL(dedisperse_kernel, dim3(grid), dim3(256), lds, s, a);
+    hipLaunchKernelGGL(dedisperse_kernel, dim3(grid), dim3(kThreads), lds, s, a);
     return hipGetLastError();
 }
 
@@ -593,6 +536,10 @@ hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in
     L.nseg = st.seg_end - st.seg_begin;
     L.items = st.items;
     L.lds_bytes = st.lds_bytes;
+    if (p->tune.persistent) {
+        L.counter = p->dev.counter + (&st - p->stages.data());
+        L.grid = p->tune.grid ? (int32_t)p->tune.grid : 8 * std::max(1, p->cus);
+    }
     L.njobs = 1;
     L.jobs[0] = JobBufs{in, out, (char *)work, in_stride};
     return launch_dedisperse_spec(L, s);

@@ -12,8 +12,15 @@
 namespace mdsm {
 
 // Dedispersion tiling (the kernel's compile-time shape).
-constexpr int kWaves = 4;            // waves per workgroup
-constexpr int kRPW = 8;              // DM trials per wave (registers)
+#ifndef MDSM_WAVES
+#define MDSM_WAVES 8
+#endif
+#ifndef MDSM_RPW
+#define MDSM_RPW 8
+#endif
+constexpr int kWaves = MDSM_WAVES;   // waves per workgroup
+constexpr int kRPW = MDSM_RPW;       // DM trials per wave (registers)
+constexpr int kThreads = 64 * kWaves;
 constexpr int kDB = kWaves * kRPW;   // DM trials per block
 constexpr int kPPL = 4;              // output samples per lane (stride 64)
 constexpr int kTile = 64 * kPPL;     // output samples per tile
@@ -64,6 +71,7 @@ struct BlockDesc {
 inline int lds_row_bytes(int dtype, int W) { return (dtype == DT_U8 || dtype == DT_U16) ? 8 * W : 4 * W; }
 inline bool int_path(int dtype) { return dtype == DT_U8 || dtype == DT_U16; }
 constexpr int kLdsBudget = 65536;  // bytes of LDS per workgroup
+constexpr int kIntChunkRows = 6;   // integer path: default channels per LDS chunk
 
 // A run of blocks tiled over the same number of output tiles; flat work
 // items of a segment are ordered tile-major (all blocks of tile 0 first).
@@ -121,6 +129,7 @@ struct KernelTables {
     SegDesc *segs = nullptr;
     int32_t *rel = nullptr;
     int32_t *abase = nullptr;
+    uint32_t *counter = nullptr;  // persistent work-queue counters, one per stage
 };
 
 // A dedispersion launch over segments already resident on the device.
@@ -140,6 +149,7 @@ struct LaunchSpec {
 struct mdsm_plan {
     mdsm_obs obs{};
     int device = 0;
+    int cus = 0;            // compute units of the device
     bool uploaded = false;  // device tables are uploaded on first use
     bool subband = false;
     uint32_t nsubs = 0;
@@ -150,6 +160,8 @@ struct mdsm_plan {
     std::vector<int32_t> rel, abase;
     std::vector<mdsm::Stage> stages;   // launched in order
     mdsm::KernelTables dev{};
+    mdsm_tuning tune{};
+    std::string shape_key;             // autotuner cache key
     void *own_work = nullptr;
     uint64_t work_bytes = 0;
     mdsm_plan_info info{};

@@ -6,8 +6,11 @@
 #include <algorithm>
 #include <cmath>
 #include <cstring>
+#include <cstdio>
 #include <functional>
 #include <limits>
+#include <map>
+#include <mutex>
 
 #include "mdsm_internal.h"
 
@@ -133,19 +136,7 @@ int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
         b.src = s.src;
         b.dst = s.dst;
         b.dtype = s.dtype;
-        {
-            const int rb = lds_row_bytes(s.dtype, b.W);
-            if (int_path(s.dtype)) {
-                // double-buffered chunks of up to 8 rows, plus two 8-row offset tables
-                b.ccr = std::max(2, std::min(8, (kLdsBudget - 2048) / (2 * rb)));
-                st.lds_bytes = std::max(st.lds_bytes, 2 * b.ccr * rb + 2 * 8 * kDB * 4);
-                b.flush = std::max(1, 65535 / std::max(1, s.maxval));
-            } else {
-                b.ccr = std::max(1, std::min(kCC, kLdsBudget / rb));
-                st.lds_bytes = std::max(st.lds_bytes, b.ccr * rb);
-                b.flush = 0;
-            }
-        }
+        b.flush = int_path(s.dtype) ? std::max(1, 65535 / std::max(1, s.maxval)) : 0;
         b.tab_off = (int32_t)p->rel.size();
         b.base_off = (int32_t)p->abase.size();
         if (p->rel.size() + (size_t)s.nrows * kDB > (size_t)std::numeric_limits<int32_t>::max())
@@ -192,6 +183,37 @@ int add_decimation(mdsm_plan *p, int k, uint64_t &work)
     return (int)p->decim.size() - 1;
 }
 
+// Channels per LDS chunk of every block and each launch's dynamic LDS, from
+// the plan's tuning: the requested rows, or fewer where a block's window is
+// too wide for the LDS budget.  Integer path: two chunk buffers
+// (double-buffered), 2..8 rows, 6 by default; float path: one, 1..16 rows,
+// as many as fit by default.
+void apply_tuning(mdsm_plan *p)
+{
+    const int budget = p->tune.lds_budget ? (int)p->tune.lds_budget : kLdsBudget;
+    for (Stage &st : p->stages) {
+        st.lds_bytes = 0;
+        for (int si = st.seg_begin; si < st.seg_end; ++si) {
+            const SegDesc &sg = p->segs[si];
+            for (int bi = sg.first_block; bi < sg.first_block + sg.nblocks; ++bi) {
+                BlockDesc &b = p->blocks[bi];
+                const int rb = lds_row_bytes(b.dtype, b.W);
+                int lds;
+                if (int_path(b.dtype)) {
+                    const int want = p->tune.chunk_rows ? (int)p->tune.chunk_rows : kIntChunkRows;
+                    b.ccr = std::max(2, std::min({8, want, budget / (2 * rb)}));
+                    lds = 2 * b.ccr * rb;
+                } else {
+                    const int want = p->tune.chunk_rows ? (int)p->tune.chunk_rows : kCC;
+                    b.ccr = std::max(1, std::min({kCC, want, budget / rb}));
+                    lds = b.ccr * rb;
+                }
+                st.lds_bytes = std::max(st.lds_bytes, lds);
+            }
+        }
+    }
+}
+
 int upload(mdsm_plan *p)
 {
     hipError_t e = hipGetDevice(&p->device);
@@ -217,9 +239,33 @@ int upload(mdsm_plan *p)
         sg.job = 0;
     }
     if ((e = up((void **)&p->dev.segs, p->segs.data(), p->segs.size() * sizeof(SegDesc))) !=
-        hipSuccess)
+            hipSuccess ||
+        (e = up((void **)&p->dev.counter, nullptr, p->stages.size() * sizeof(uint32_t))) !=
+            hipSuccess)
         return hip_fail(e, "uploading plan tables");
+    if ((e = hipDeviceGetAttribute(&p->cus, hipDeviceAttributeMultiprocessorCount, p->device)) !=
+        hipSuccess)
+        return hip_fail(e, "hipDeviceGetAttribute");
     return MDSM_OK;
+}
+
+// Launch shapes chosen by the autotuner, per plan shape, for this process.
+std::mutex g_tune_mu;
+std::map<std::string, mdsm_tuning> g_tune_cache;
+
+std::string shape_key_of(const mdsm_obs &o, const std::string &plan_part)
+{
+    char buf[256];
+    std::snprintf(buf, sizeof buf, "%u|%.9g|%.9g|%.9g|%u|%d|%d|", o.nchans, (double)o.fch1,
+                  (double)o.foff, (double)o.tsamp, o.nsamp_in, o.dtype, o.shift_rule);
+    return std::string(buf) + plan_part;
+}
+
+void adopt_cached_tuning(mdsm_plan *p)
+{
+    std::lock_guard<std::mutex> g(g_tune_mu);
+    auto it = g_tune_cache.find(p->shape_key);
+    if (it != g_tune_cache.end()) p->tune = it->second;
 }
 
 void finish_info(mdsm_plan *p, double adds, double bytes, double dcs)
@@ -354,6 +400,18 @@ extern "C" int mdsm_plan_create(const mdsm_obs *obs, const mdsm_range *ranges, u
         }
     }
     p->stages.push_back(st);
+    {
+        std::string part = "B";
+        for (uint32_t r = 0; r < nranges; ++r) {
+            char buf[128];
+            std::snprintf(buf, sizeof buf, "|%.9g,%.9g,%u,%u", (double)ranges[r].lowdm,
+                          (double)ranges[r].dmstep, ranges[r].ndms, ranges[r].downsample);
+            part += buf;
+        }
+        p->shape_key = shape_key_of(*obs, part);
+    }
+    adopt_cached_tuning(p);
+    apply_tuning(p);
     p->info.ndms_total = dm_row;
     p->info.maxshift = maxshift;
     p->info.out_elems = out_elems;
@@ -548,6 +606,20 @@ extern "C" int mdsm_plan_create_subband(const mdsm_obs *obs, uint32_t nsubs, con
     }
     p->stages.push_back(s1);
     p->stages.push_back(s2);
+    {
+        std::string part = "S" + std::to_string(nsubs);
+        for (uint32_t q = 0; q < npasses; ++q) {
+            const mdsm_pass &P = passes[q];
+            char buf[192];
+            std::snprintf(buf, sizeof buf, "|%.9g,%.9g,%.9g,%.9g,%u,%u,%u,%u", (double)P.lowdm,
+                          (double)P.highdm, (double)P.dmstep, (double)P.sub_dmstep, P.binsize,
+                          P.ndms, P.ncalls, P.calldms);
+            part += buf;
+        }
+        p->shape_key = shape_key_of(*obs, part);
+    }
+    adopt_cached_tuning(p);
+    apply_tuning(p);
     p->info.ndms_total = dm_row;
     p->info.maxshift = maxshift;
     p->info.out_elems = out_elems;
@@ -571,6 +643,7 @@ extern "C" void mdsm_plan_destroy(mdsm_plan *p)
     if (p->dev.segs) (void)hipFree(p->dev.segs);
     if (p->dev.rel) (void)hipFree(p->dev.rel);
     if (p->dev.abase) (void)hipFree(p->dev.abase);
+    if (p->dev.counter) (void)hipFree(p->dev.counter);
     if (p->own_work) (void)hipFree(p->own_work);
     if (sw) (void)hipSetDevice(cur);
     delete p;
@@ -660,5 +733,105 @@ extern "C" int mdsm_dedisperse(mdsm_plan *p, const void *d_in, int64_t in_stride
         e = launch_dedisperse(p, st, d_in, in_stride, d_out, work, s);
         if (e != hipSuccess) return hip_fail(e, "dedisperse launch");
     }
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_plan_set_tuning(mdsm_plan *p, const mdsm_tuning *t)
+{
+    if (!p || !t) return fail(MDSM_ERR_INVALID, "NULL argument");
+    if (t->chunk_rows > (uint32_t)kCC) return fail(MDSM_ERR_INVALID, "chunk_rows must be <= 16");
+    if (t->lds_budget > (uint32_t)kLdsBudget)
+        return fail(MDSM_ERR_INVALID, "lds_budget must be <= 65536");
+    if (t->lds_budget && t->lds_budget < 4096)
+        return fail(MDSM_ERR_INVALID, "lds_budget must be 0 or >= 4096");
+    if (t->persistent > 1) return fail(MDSM_ERR_INVALID, "persistent must be 0 or 1");
+    if (t->grid > (1u << 20)) return fail(MDSM_ERR_INVALID, "grid must be <= 2^20");
+    p->tune = *t;
+    apply_tuning(p);
+    if (p->uploaded) {
+        // only chunk sizes changed: refresh the block table in place
+        int cur = 0;
+        hipError_t e = hipGetDevice(&cur);
+        if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+        if (cur != p->device) return fail(MDSM_ERR_STATE, "plan belongs to another device");
+        e = hipMemcpy(p->dev.blocks, p->blocks.data(), p->blocks.size() * sizeof(BlockDesc),
+                      hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "updating the block table");
+    }
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_plan_get_tuning(const mdsm_plan *p, mdsm_tuning *t)
+{
+    if (!p || !t) return fail(MDSM_ERR_INVALID, "NULL argument");
+    *t = p->tune;
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_plan_autotune(mdsm_plan *p, const void *d_in, int64_t in_stride, float *d_out,
+                                  void *d_work, void *stream, uint32_t reps, mdsm_tuning *best)
+{
+    if (!p || !d_in || !d_out) return fail(MDSM_ERR_INVALID, "NULL argument");
+    if (reps < 1 || reps > 1000) return fail(MDSM_ERR_INVALID, "reps must be 1..1000");
+    hipStream_t s = (hipStream_t)stream;
+    std::vector<mdsm_tuning> cand;
+    bool any_int = false, any_float = false;
+    for (const BlockDesc &b : p->blocks) (int_path(b.dtype) ? any_int : any_float) = true;
+    std::vector<uint32_t> rows;
+    if (any_int) rows.insert(rows.end(), {2, 3, 4, 6, 8});
+    if (any_float) rows.insert(rows.end(), {4, 8, 12, 16});
+    std::sort(rows.begin(), rows.end());
+    rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+    for (uint32_t per = 0; per <= 1; ++per)
+        for (uint32_t r : rows) {
+            mdsm_tuning t{};
+            t.chunk_rows = r;
+            t.persistent = per;
+            cand.push_back(t);
+        }
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return hip_fail(e, "hipEventCreate");
+    }
+    const mdsm_tuning saved = p->tune;
+    float best_ms = std::numeric_limits<float>::max();
+    mdsm_tuning best_t = saved;
+    int rc = MDSM_OK;
+    for (const mdsm_tuning &t : cand) {
+        if ((rc = mdsm_plan_set_tuning(p, &t))) break;
+        if ((rc = mdsm_dedisperse(p, d_in, in_stride, d_out, d_work, stream))) break;  // warm-up
+        if ((e = hipEventRecord(e0, s)) != hipSuccess) {
+            rc = hip_fail(e, "hipEventRecord");
+            break;
+        }
+        for (uint32_t i = 0; i < reps && !rc; ++i)
+            rc = mdsm_dedisperse(p, d_in, in_stride, d_out, d_work, stream);
+        if (rc) break;
+        float ms = 0.0f;
+        if ((e = hipEventRecord(e1, s)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
+            (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) {
+            rc = hip_fail(e, "timing a candidate");
+            break;
+        }
+        if (ms < best_ms) {
+            best_ms = ms;
+            best_t = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        (void)mdsm_plan_set_tuning(p, &saved);
+        return rc;
+    }
+    if ((rc = mdsm_plan_set_tuning(p, &best_t))) return rc;
+    {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        g_tune_cache[p->shape_key] = best_t;
+    }
+    if (best) *best = best_t;
     return MDSM_OK;
 }
