@@ -26,6 +26,10 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 VALU_ADD_PEAK_T = 39.3         # unpacked fp32 adds/s (256 CU x 2.4 GHz x 64), T/s
+# LDS read peak: 128 B/clk/CU x 256 CUs x 2.4 GHz.  A ds_read_b64 probe on
+# MI355X reaches 75.4 TB/s (122.8 B/clk/CU) and ds_read_b128 121.5 TB/s.
+LDS_PEAK_TBS = 78.6
+PROFILE_DIR = os.path.join(os.path.dirname(os.path.abspath(__file__)), "profiles")
 
 WORKLOADS = {
     "cfg2": dict(nchans=1024, nsamp=1 << 20, fch1=1500.0, foff=-300.0 / 1024, tsamp=64e-6,
@@ -299,6 +303,14 @@ def main():
     alg_bytes = dedisperse_bytes(plan, w)
     achieved_gbs = alg_bytes / avg_launch_s / 1e9
     adds_per_launch = info.dm_chan_samples  # one float add per DM*chan*sample
+    # HBM traffic of the dedispersion kernel from the committed rocprofv3
+    # FETCH_SIZE/WRITE_SIZE summary of this workload (copy-calibrated)
+    from mdsm_amd.profile import read_traffic
+    prof = os.path.join(PROFILE_DIR, f"dedisperse_kernel_{args.workload}.md")
+    tr = read_traffic(prof)
+    # LDS: the integer path reads one uint16 per (DM, channel, sample), the
+    # float path one float32
+    lds_bytes = info.dm_chan_samples * (2.0 if w["dtype"] == "u8" else 4.0)
     out = {
         "metric": "DM*chan*samples/s",
         "value": value,
@@ -317,9 +329,16 @@ def main():
                    "nsamp": w["nsamp"], "ndms": int(info.ndms_total),
                    "dm_chan_samples_per_step": info.dm_chan_samples},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
+                     "traffic": tr["traffic_bytes_per_launch"] if tr else None,
+                     "traffic_source": os.path.relpath(prof, os.path.dirname(PROFILE_DIR))
+                     if tr else None,
                      "kernel": "dedisperse_kernel", "avg_launch_ms": avg_launch_s * 1e3,
                      "alg_bytes_per_launch": alg_bytes},
+        "lds": {"bytes_per_launch": lds_bytes,
+                "achieved_tbs": lds_bytes / avg_launch_s / 1e12, "peak_tbs": LDS_PEAK_TBS,
+                "frac": lds_bytes / avg_launch_s / 1e12 / LDS_PEAK_TBS,
+                "note": "binding resource: LDS gathers, one per DM*chan*sample"},
         "valu": {"adds_per_launch": adds_per_launch,
                  "achieved_tadds": adds_per_launch / avg_launch_s / 1e12,
                  "peak_tadds": VALU_ADD_PEAK_T,
