@@ -199,6 +199,28 @@ int mdsm_decimate(const void *d_in, int32_t dtype, uint32_t nchans,
 /* RFI clipper and single-pulse detector                               */
 /* ------------------------------------------------------------------ */
 
+/* Corner turn of a time-major block into channel-major rows:
+ * d_out[c][out_col0 + t] = d_in[t][invert ? nchans-1-c : c], t < nsamp,
+ * c < nchans (in_stride / out_stride in elements).  dtype_out is dtype_in
+ * (storage kept, e.g. uint8 for the integer path) or MDSM_F32 (unpacked to
+ * float as the reference does, (float)byte for uint8).  Replaces the host
+ * corner-turn loops of src/pelican-mdsm/modules/MdsmModule.cpp:135-151
+ * (invertChannels: :27,137,145) and src/MDSM.cpp:100-120, and the uint8
+ * unpack of src/mdsm/file_handler.cpp:255-264. */
+int mdsm_corner_turn(const void *d_in, int32_t dtype_in, uint32_t nsamp,
+                     uint32_t nchans, int64_t in_stride, int32_t invert,
+                     void *d_out, int32_t dtype_out, int64_t out_stride,
+                     int64_t out_col0, void *stream);
+
+/* In-place detrend and normalise of nrows rows of n floats: each segment of
+ * seg_len samples (the last one may be shorter) has its least-squares line
+ * a + b*x removed, x = i - L/2 + 0.5 centred on the segment, and is divided
+ * by the RMS of the residual.  Sums are formed in FP64.  A segment whose
+ * residual RMS is zero is set to zero (the reference divides by it).
+ * Replaces detrend_normalise (src/prototypes/detrend.cu:12-100). */
+int mdsm_detrend(float *d_x, uint32_t nrows, uint32_t n, int64_t stride,
+                 uint32_t seg_len, void *stream);
+
 /* Per-channel robust clip of uint8 samples [t0, t0+n) of each channel:
  * med = lower median, mad = lower median of |x-med|, samples with
  * |x-med| > (nsigma*1.4826f)*mad are replaced by med.  d_out may equal d_in.

@@ -139,6 +139,8 @@ def lib():
         "mdsm_timing_enable": [C.c_int],
         "mdsm_timing_reset": [],
         "mdsm_timing_get": [C.c_char_p, C.POINTER(f64), C.POINTER(u64)],
+        "mdsm_corner_turn": [vp, i32, u32, u32, i64, i32, vp, i32, i64, i64, vp],
+        "mdsm_detrend": [vp, u32, u32, i64, u32, vp],
         "mdsm_plan_set_tuning": [vp, C.POINTER(Tuning)],
         "mdsm_plan_get_tuning": [vp, C.POINTER(Tuning)],
         "mdsm_plan_autotune": [vp, vp, i64, vp, vp, vp, u32, C.POINTER(Tuning)],
@@ -427,6 +429,23 @@ def clip_u8(d_in, in_stride, nchans, t0, n, nsigma, d_out, out_stride, d_med=Non
     g = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
     check(lib().mdsm_clip_u8(g(d_in), in_stride, nchans, t0, n, nsigma, g(d_out), out_stride,
                              g(d_med), g(d_mad), g(d_ncl), _sh(stream)))
+
+
+def corner_turn(d_in, dtype_in, nsamp, nchans, in_stride, d_out, out_stride, out_col0=0,
+                invert=False, dtype_out=None, stream=None):
+    """Time-major [nsamp][in_stride] -> channel-major rows out[c][out_col0 + t]."""
+    g = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
+    di = DTYPES[dtype_in] if isinstance(dtype_in, str) else int(dtype_in)
+    do = di if dtype_out is None else (DTYPES[dtype_out] if isinstance(dtype_out, str)
+                                       else int(dtype_out))
+    check(lib().mdsm_corner_turn(g(d_in), di, nsamp, nchans, in_stride, 1 if invert else 0,
+                                 g(d_out), do, out_stride, out_col0, _sh(stream)))
+
+
+def detrend(d_x, nrows, n, stride, seg_len, stream=None):
+    """In-place per-segment detrend and normalise of float rows."""
+    g = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
+    check(lib().mdsm_detrend(g(d_x), nrows, n, stride, seg_len, _sh(stream)))
 
 
 def detect_boxcar(d_x, ndms, n, stride, widths, thr, d_cands, cap, d_count, d_mean, d_sigma,

@@ -245,8 +245,10 @@ __device__ __forceinline__ void store_chunk(const PreRegs<DT> &pre, uint2 *buf, 
 
 // Integer path: chunks of ccr <= 8 channels are double-buffered in LDS; the
 // next chunk's global loads are in flight (in registers) while the current
-// chunk is summed.  Each wave reads its 8 trials' offsets of a channel pair
-// with two scalar loads and issues 16 ds_read_b64 before the adds.
+// chunk is summed.  Each wave reads its kRPW trials' offsets of a channel
+// pair with scalar loads; one offset serves the lane's kGroups 4-sample
+// groups (256 samples apart: the same ds_read_b64 address plus an immediate
+// 512-byte step).
 template <int DT>
 __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
                                  int lane, uint2 *lds)
@@ -261,23 +263,27 @@ __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, 
     const size_t bufsz = (size_t)ccr * 4 * K;  // uint2 per chunk buffer
     const int nch = (bd.nrows + ccr - 1) / ccr;
 
-    float f[kRPW][4];
-    uint32_t ae[kRPW], ao[kRPW];
+    float f[kRPW][kGroups][4];
+    uint32_t ae[kRPW][kGroups], ao[kRPW][kGroups];
 #pragma unroll
-    for (int r = 0; r < kRPW; ++r) {
-        ae[r] = ao[r] = 0u;
+    for (int r = 0; r < kRPW; ++r)
 #pragma unroll
-        for (int j = 0; j < 4; ++j) f[r][j] = 0.0f;
-    }
+        for (int g = 0; g < kGroups; ++g) {
+            ae[r][g] = ao[r][g] = 0u;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) f[r][g][j] = 0.0f;
+        }
     auto flush = [&]() {
 #pragma unroll
-        for (int r = 0; r < kRPW; ++r) {
-            f[r][0] += (float)(ae[r] & 0xffffu);
-            f[r][2] += (float)(ae[r] >> 16);
-            f[r][1] += (float)(ao[r] & 0xffffu);
-            f[r][3] += (float)(ao[r] >> 16);
-            ae[r] = ao[r] = 0u;
-        }
+        for (int r = 0; r < kRPW; ++r)
+#pragma unroll
+            for (int g = 0; g < kGroups; ++g) {
+                f[r][g][0] += (float)(ae[r][g] & 0xffffu);
+                f[r][g][2] += (float)(ae[r][g] >> 16);
+                f[r][g][1] += (float)(ao[r][g] & 0xffffu);
+                f[r][g][3] += (float)(ao[r][g] >> 16);
+                ae[r][g] = ao[r][g] = 0u;
+            }
     };
     // sum rows [0, cc) of the staged chunk into the packed accumulators
     auto sum_chunk = [&](const char *lane_base, const int32_t *grow, int cc) {
@@ -292,17 +298,21 @@ __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, 
             }
             const char *rowi = lane_base + i * rb;
             const char *rowj = rowi + rb;
-            uint2 x[kRPW], y[kRPW];
+            uint2 x[kRPW][kGroups], y[kRPW][kGroups];
 #pragma unroll
-            for (int r = 0; r < kRPW; ++r) {
-                x[r] = *(const uint2 *)(rowi + oi[r]);
-                y[r] = *(const uint2 *)(rowj + oj[r]);
-            }
+            for (int r = 0; r < kRPW; ++r)
 #pragma unroll
-            for (int r = 0; r < kRPW; ++r) {
-                ae[r] = ae[r] + x[r].x + y[r].x;
-                ao[r] = ao[r] + x[r].y + y[r].y;
-            }
+                for (int g = 0; g < kGroups; ++g) {
+                    x[r][g] = *(const uint2 *)(rowi + oi[r] + 512 * g);
+                    y[r][g] = *(const uint2 *)(rowj + oj[r] + 512 * g);
+                }
+#pragma unroll
+            for (int r = 0; r < kRPW; ++r)
+#pragma unroll
+                for (int g = 0; g < kGroups; ++g) {
+                    ae[r][g] = ae[r][g] + x[r][g].x + y[r][g].x;
+                    ao[r][g] = ao[r][g] + x[r][g].y + y[r][g].y;
+                }
         }
         if (i < cc) {
             int oi[kRPW];
@@ -311,11 +321,13 @@ __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, 
             for (int r = 0; r < kRPW; ++r) oi[r] = ri[r];
             const char *rowi = lane_base + i * rb;
 #pragma unroll
-            for (int r = 0; r < kRPW; ++r) {
-                const uint2 x = *(const uint2 *)(rowi + oi[r]);
-                ae[r] += x.x;
-                ao[r] += x.y;
-            }
+            for (int r = 0; r < kRPW; ++r)
+#pragma unroll
+                for (int g = 0; g < kGroups; ++g) {
+                    const uint2 x = *(const uint2 *)(rowi + oi[r] + 512 * g);
+                    ae[r][g] += x.x;
+                    ao[r][g] += x.y;
+                }
         }
     };
 
@@ -351,22 +363,25 @@ __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, 
 
     float *ob = (bd.dst == BUF_OUT ? a.out : (float *)a.work) + bd.out_off;
     const float kf = (float)bd.k;
-    const int t = t0 + 4 * lane;
 #pragma unroll
     for (int r = 0; r < kRPW; ++r) {
         const int j = wave * kRPW + r;
         if (j >= bd.ntrials) continue;
-        float v[4];
-#pragma unroll
-        for (int q = 0; q < 4; ++q) v[q] = bd.k > 1 ? __fdiv_rn(f[r][q], kf) : f[r][q];
         float *o = ob + (int64_t)j * bd.out_trial_stride;
         const bool al = (((uintptr_t)(o + t0)) & 15) == 0;
-        if (al && t + 3 < bd.nout) {
-            *(float4 *)(o + t) = make_float4(v[0], v[1], v[2], v[3]);
-        } else {
 #pragma unroll
-            for (int q = 0; q < 4; ++q)
-                if (t + q < bd.nout) o[t + q] = v[q];
+        for (int g = 0; g < kGroups; ++g) {
+            const int t = t0 + 256 * g + 4 * lane;
+            float v[4];
+#pragma unroll
+            for (int q = 0; q < 4; ++q) v[q] = bd.k > 1 ? __fdiv_rn(f[r][g][q], kf) : f[r][g][q];
+            if (al && t + 3 < bd.nout) {
+                *(float4 *)(o + t) = make_float4(v[0], v[1], v[2], v[3]);
+            } else {
+#pragma unroll
+                for (int q = 0; q < 4; ++q)
+                    if (t + q < bd.nout) o[t + q] = v[q];
+            }
         }
     }
 }

@@ -22,8 +22,12 @@ constexpr int kWaves = MDSM_WAVES;   // waves per workgroup
 constexpr int kRPW = MDSM_RPW;       // DM trials per wave (registers)
 constexpr int kThreads = 64 * kWaves;
 constexpr int kDB = kWaves * kRPW;   // DM trials per block
-constexpr int kPPL = 4;              // output samples per lane (stride 64)
-constexpr int kTile = 64 * kPPL;     // output samples per tile
+#ifndef MDSM_GROUPS
+#define MDSM_GROUPS 1
+#endif
+constexpr int kGroups = MDSM_GROUPS; // integer path: 4-sample groups per lane, 256 apart
+constexpr int kPPL = 4 * kGroups;    // output samples per lane
+constexpr int kTile = 256 * kGroups; // output samples per tile
 constexpr int kCC = 16;              // channels per LDS chunk
 constexpr int kMaxW = 1024;          // widest LDS row (floats)
 

@@ -63,6 +63,10 @@ def lib():
         L.orc_boxcar_detect.argtypes = [vp, i32, i32, i64, vp, i32, f64, vp, i32, vp, vp]
         L.orc_median_of_five.restype = f32
         L.orc_median_of_five.argtypes = [f32] * 5
+        L.orc_corner_turn_f32.restype = None
+        L.orc_corner_turn_f32.argtypes = [vp, i32, i32, i64, i32, vp, i64]
+        L.orc_detrend_f64.restype = None
+        L.orc_detrend_f64.argtypes = [vp, i32, i32, i64, i32, vp]
         _lib = L
     return _lib
 
@@ -212,6 +216,23 @@ def boxcar_detect(x, widths, thr, cap=1 << 20):
 
 def median_of_five(a, b, c, d, e):
     return lib().orc_median_of_five(a, b, c, d, e)
+
+
+def corner_turn_f32(x_tm, invert=False):
+    """Time-major [nsamp][nchans] -> channel-major [nchans][nsamp] float32."""
+    x = np.ascontiguousarray(x_tm, np.float32)
+    T, Cn = x.shape
+    out = np.empty((Cn, T), np.float32)
+    lib().orc_corner_turn_f32(_p(x), T, Cn, Cn, 1 if invert else 0, _p(out), T)
+    return out
+
+
+def detrend_f64(x, seg_len):
+    x = np.ascontiguousarray(x, np.float32)
+    R, N = x.shape
+    out = np.empty((R, N), np.float64)
+    lib().orc_detrend_f64(_p(x), R, N, N, int(seg_len), _p(out))
+    return out
 
 
 # ---------------------------------------------------------------- reference build

@@ -519,3 +519,61 @@ float orc_median_of_five(float a, float b, float c, float d, float e)
     }
     return v[2];
 }
+
+/* ------------------------------------------------------------------ */
+/* Input corner turn                                                   */
+/* ------------------------------------------------------------------ */
+
+/* out[c][t] = in[t][invert ? C-1-c : c]: the copy loop of
+ * src/pelican-mdsm/modules/MdsmModule.cpp:135-151 (subband and channel
+ * indices reversed together when invertChannels is set, :137,145) and
+ * src/MDSM.cpp:100-120. */
+void orc_corner_turn_f32(const float *in, int nsamp, int nchans, int64_t in_stride,
+                         int invert, float *out, int64_t out_stride)
+{
+    for (int t = 0; t < nsamp; t++)
+        for (int c = 0; c < nchans; c++) {
+            int src = invert ? nchans - 1 - c : c;
+            out[(int64_t)c * out_stride + t] = in[(int64_t)t * in_stride + src];
+        }
+}
+
+/* ------------------------------------------------------------------ */
+/* Detrend and normalise                                               */
+/* ------------------------------------------------------------------ */
+
+/* Per row, per segment of seg_len samples (a shorter last segment uses its
+ * own length L): x_i = -L/2 + 0.5 + i, a = sum(y)/L, b = sum(x y)/sum(x x),
+ * r_i = y_i - (a + b x_i), out_i = r_i / sqrt(sum(r^2)/L)
+ * (src/prototypes/detrend.cu:19-31,52-56,62-75,82-96), in FP64.  A zero
+ * residual RMS gives zeros (the reference divides by it). */
+void orc_detrend_f64(const float *x, int nrows, int n, int64_t stride, int seg_len,
+                     double *out)
+{
+    for (int r = 0; r < nrows; r++) {
+        const float *row = x + (int64_t)r * stride;
+        double *o = out + (int64_t)r * n;
+        for (int s0 = 0; s0 < n; s0 += seg_len) {
+            int L = n - s0 < seg_len ? n - s0 : seg_len;
+            double sy = 0.0, sxy = 0.0, sxx = 0.0;
+            for (int i = 0; i < L; i++) {
+                double xi = -(double)L / 2.0 + 0.5 + (double)i;
+                double y = row[s0 + i];
+                sy += y;
+                sxy += xi * y;
+                sxx += xi * xi;
+            }
+            double a = sy / (double)L;
+            double b = sxx > 0.0 ? sxy / sxx : 0.0;
+            double ss = 0.0;
+            for (int i = 0; i < L; i++) {
+                double xi = -(double)L / 2.0 + 0.5 + (double)i;
+                double v = (double)row[s0 + i] - (a + b * xi);
+                o[s0 + i] = v;
+                ss += v * v;
+            }
+            double rms = sqrt(ss / (double)L);
+            for (int i = 0; i < L; i++) o[s0 + i] = rms > 0.0 ? o[s0 + i] / rms : 0.0;
+        }
+    }
+}

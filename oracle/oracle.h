@@ -36,6 +36,12 @@
  *   orc_median_of_five
  *       all 5^5 tuples against the reference's own medianOfFive built under
  *       oracle/_ref and against sort.
+ *   orc_corner_turn_f32
+ *       numpy transpose (and channel reversal) of the same block.
+ *   orc_detrend_f64
+ *       numpy least-squares line fit (polyfit) on tiny inputs; invariants
+ *       of the output (zero mean, zero slope, unit RMS per segment); a pure
+ *       line detrends to zero.
  */
 #ifndef MDSM_ORACLE_H
 #define MDSM_ORACLE_H
@@ -111,6 +117,14 @@ int orc_boxcar_detect(const float *x, int ndms, int n, int64_t stride,
 
 /* --- reference in-path detector (median of five) --------------------- */
 float orc_median_of_five(float a, float b, float c, float d, float e);
+
+/* Corner turn of a time-major float block (MdsmModule.cpp:135-151). */
+void orc_corner_turn_f32(const float *in, int nsamp, int nchans, int64_t in_stride,
+                         int invert, float *out, int64_t out_stride);
+
+/* Detrend and normalise, FP64 (src/prototypes/detrend.cu:12-100). */
+void orc_detrend_f64(const float *x, int nrows, int n, int64_t stride, int seg_len,
+                     double *out);
 
 #ifdef __cplusplus
 }

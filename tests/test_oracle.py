@@ -341,3 +341,40 @@ def test_median_of_five_matches_reference_build():
     R = oracle.ref()
     for t in itertools.product(range(5), repeat=5):
         assert R.median_of_five(*map(float, t)) == oracle.median_of_five(*map(float, t))
+
+
+# ---------------------------------------------------------------- corner turn, detrend
+@pytest.mark.parametrize("invert", [False, True])
+def test_corner_turn_is_numpy_transpose(invert):
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((37, 23)).astype(np.float32)   # [t][c]
+    got = oracle.corner_turn_f32(x, invert)
+    exp = x.T[::-1] if invert else x.T
+    assert np.array_equal(got, exp)
+
+
+def test_detrend_matches_polyfit_and_invariants():
+    rng = np.random.default_rng(6)
+    L = 64
+    t = np.arange(3 * L + 17, dtype=np.float64)
+    rows = np.stack([0.3 * t - 5 + rng.standard_normal(t.size),
+                     100 + 0.01 * t ** 1.5 + rng.standard_normal(t.size)]).astype(np.float32)
+    got = oracle.detrend_f64(rows, L)
+    for r in range(rows.shape[0]):
+        for s0 in range(0, t.size, L):
+            seg = rows[r, s0:s0 + L].astype(np.float64)
+            n = seg.size
+            x = np.arange(n) - n / 2 + 0.5
+            b, a = np.polyfit(x, seg, 1)            # independent least-squares fit
+            res = seg - (a + b * x)
+            exp = res / np.sqrt(np.mean(res ** 2))
+            g = got[r, s0:s0 + n]
+            assert np.allclose(g, exp, rtol=1e-9, atol=1e-9)
+            assert abs(g.mean()) < 1e-9 and abs((g * x).sum()) < 1e-6
+            assert abs(np.mean(g ** 2) - 1.0) < 1e-9
+
+
+def test_detrend_pure_line_is_zero():
+    x = (np.arange(256, dtype=np.float32) * 0.25 - 3.0)[None, :]
+    got = oracle.detrend_f64(x, 128)
+    assert np.all(got == 0.0)
