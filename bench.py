@@ -37,6 +37,13 @@ WORKLOADS = {
                                      (500.0, 1.0, 500, 4)]),
     "cfg1": dict(nchans=256, nsamp=65536, fch1=1500.0, foff=-0.390625, tsamp=64e-6,
                  dtype="f32", ranges=[(0.0, 0.2, 512, 1)]),
+    # two-stage subband: 4096 ch, 1100-1500 MHz, 2^19 f16 samples, 4096 DMs over
+    # 0-1500 as 64 coarse DMs x 64 fine, 64 subbands (lowdm = half a call's span,
+    # so the calls tile 0-1500 without overlap)
+    "cfg3": dict(nchans=4096, nsamp=1 << 19, fch1=1500.0, foff=-400.0 / 4096, tsamp=54.6e-6,
+                 dtype="f16", nsubs=64,
+                 passes=[dict(lowdm=64 * (1500.0 / 4096) / 2, dmstep=1500.0 / 4096,
+                              sub_dmstep=64 * (1500.0 / 4096), ncalls=64, calldms=64)]),
     # serving: 32 beams of 1024-ch uint8 blocks of 2^16 samples, 2048 DMs over 0-1000
     "cfg5": dict(nchans=1024, nsamp=1 << 16, fch1=1500.0, foff=-300.0 / 1024, tsamp=64e-6,
                  dtype="u8", ranges=[(0.0, 1000.0 / 2048, 2048, 1)], beams=32,
@@ -157,8 +164,23 @@ def serve(M, w, args):
 
 
 
+def make_plan(M, w):
+    if "passes" in w:
+        return M.Plan.subband(w["nchans"], w["fch1"], w["foff"], w["tsamp"], w["nsamp"],
+                              w["nsubs"], w["passes"], dtype=w["dtype"])
+    return M.Plan.brute(w["nchans"], w["fch1"], w["foff"], w["tsamp"], w["nsamp"], w["dtype"],
+                        w["ranges"])
+
+
 def make_input(M, w, seed=20240601):
     rng = np.random.default_rng(seed)
+    if w["dtype"] == "f16":
+        pulses = [(float(rng.uniform(100, 1400)), int(rng.integers(10_000, w["nsamp"] - 60_000)),
+                   int(rng.integers(1, 33)), 0.5) for _ in range(3)]
+        x = M.synth_filterbank(w["nchans"], w["nsamp"], "f16", seed=seed, fch1=w["fch1"],
+                               foff=w["foff"], tsamp=w["tsamp"], gain=(0.5, 2.0),
+                               pulses=pulses, nthreads=16)
+        return x, pulses
     if w["dtype"] == "u8":
         pulses = [(float(rng.uniform(10, 900)), int(rng.integers(50_000, w["nsamp"] - 500_000)),
                    int(rng.integers(1, 33)), 1.0) for _ in range(5)]
@@ -173,9 +195,13 @@ def make_input(M, w, seed=20240601):
 
 
 def dedisperse_bytes(plan, w):
-    """Algorithmic HBM bytes of one dedispersion launch: every input sample it
-    reads once (the raw block for k=1 ranges, each decimated copy for k>1),
-    every output sample written once, plus the shift tables."""
+    """Algorithmic HBM bytes of the dedispersion launches of one step: every
+    input sample read once (the raw block for k=1 ranges, each decimated copy
+    for k>1), every output sample written once; for the two-stage subband
+    plan the plan's own count (input, stage-1 series written and read back,
+    output, tables)."""
+    if "passes" in w:
+        return float(plan.info.bytes)
     esize = {"u8": 1, "f16": 2, "f32": 4}[w["dtype"]]
     C, S = w["nchans"], w["nsamp"]
     b = 0.0
@@ -271,8 +297,7 @@ def main():
         return
     print(f"[bench] workload {args.workload}: generating input", file=sys.stderr, flush=True)
     x, pulses = make_input(M, w)
-    plan = M.Plan.brute(w["nchans"], w["fch1"], w["foff"], w["tsamp"], w["nsamp"], w["dtype"],
-                        w["ranges"])
+    plan = make_plan(M, w)
     info = plan.info
     stream = M.Stream()
     din = M.DeviceBuffer.from_host(x)
@@ -299,10 +324,12 @@ def main():
 
     ms_per_step = wall * 1e3 / args.steps
     value = info.dm_chan_samples * args.steps / wall
-    avg_launch_s = ded_ms / max(ded_n, 1) / 1e3
+    # one dedispersion launch per step (two for the subband plan): the
+    # roofline is per step's launches
+    launches_per_step = max(ded_n, 1) / args.steps
+    avg_launch_s = ded_ms / max(ded_n, 1) / 1e3 * launches_per_step
     alg_bytes = dedisperse_bytes(plan, w)
     achieved_gbs = alg_bytes / avg_launch_s / 1e9
-    adds_per_launch = info.dm_chan_samples  # one float add per DM*chan*sample
     # HBM traffic of the dedispersion kernel from the committed rocprofv3
     # FETCH_SIZE/WRITE_SIZE summary of this workload (copy-calibrated)
     from mdsm_amd.profile import read_traffic
@@ -310,7 +337,10 @@ def main():
     tr = read_traffic(prof)
     # LDS: the integer path reads one uint16 per (DM, channel, sample), the
     # float path one float32
-    lds_bytes = info.dm_chan_samples * (2.0 if w["dtype"] == "u8" else 4.0)
+    dec_adds = sum(w["nchans"] * (w["nsamp"] // k) * (k - 1)
+                   for k in sorted({R.downsample for R in plan.ranges}) if k > 1)
+    adds_per_launch = info.adds - dec_adds       # one LDS read per dedispersion add
+    lds_bytes = adds_per_launch * (2.0 if w["dtype"] == "u8" else 4.0)
     out = {
         "metric": "DM*chan*samples/s",
         "value": value,
@@ -325,7 +355,9 @@ def main():
         "dtype": w["dtype"] + "->f32",
         "data": "synthetic (seeded Gaussian filterbank with injected pulses)",
         "config": {"workload": f"{args.workload}: {w['nchans']} ch x {w['nsamp']} samples "
-                   f"{w['dtype']}, ranges {w['ranges']}", "nchans": w["nchans"],
+                   f"{w['dtype']}, " + (f"ranges {w['ranges']}" if "ranges" in w else
+                                        f"{w['nsubs']} subbands, passes {w['passes']}"),
+                   "nchans": w["nchans"],
                    "nsamp": w["nsamp"], "ndms": int(info.ndms_total),
                    "dm_chan_samples_per_step": info.dm_chan_samples},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
@@ -346,7 +378,7 @@ def main():
         "kernels_ms_per_step": {"dedisperse": ded_ms / args.steps,
                                 "decimate": dec_ms / args.steps},
     }
-    if not args.no_cpu:
+    if not args.no_cpu and "ranges" in w:
         print("[bench] CPU baseline", file=sys.stderr, flush=True)
         ndm0 = plan.ranges[0].ndms
         gpu_rows = dout.download(np.float32, (min(args.cpu_dms, ndm0), plan.ranges[0].nout))

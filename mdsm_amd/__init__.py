@@ -250,6 +250,17 @@ class DeviceBuffer:
             raise ValueError("upload exceeds buffer")
         check(lib().mdsm_memcpy(self.ptr + offset, arr.ctypes.data, arr.nbytes, H2D, _sh(stream)))
 
+    def download_rows(self, dtype, nrows: int, row_elems: int, pitch_elems: int,
+                      offset: int = 0) -> np.ndarray:
+        """Download nrows rows of row_elems elements spaced pitch_elems apart."""
+        out = np.empty((nrows, row_elems), dtype)
+        es = out.itemsize
+        if offset + ((nrows - 1) * pitch_elems + row_elems) * es > self.nbytes:
+            raise ValueError("download exceeds buffer")
+        check(lib().mdsm_memcpy_2d(out.ctypes.data, row_elems * es, self.ptr + offset,
+                                   pitch_elems * es, row_elems * es, nrows, D2H, None))
+        return out
+
     def download(self, dtype, shape, offset: int = 0) -> np.ndarray:
         out = np.empty(shape, dtype)
         if offset + out.nbytes > self.nbytes:

@@ -37,8 +37,7 @@ def _steps(workload: str, steps: int) -> None:
     import bench
     w = bench.WORKLOADS[workload]
     x, _ = bench.make_input(M, w)
-    plan = M.Plan.brute(w["nchans"], w["fch1"], w["foff"], w["tsamp"], w["nsamp"], w["dtype"],
-                        w["ranges"])
+    plan = bench.make_plan(M, w)
     din = M.DeviceBuffer.from_host(x)
     dout = M.DeviceBuffer(plan.info.out_elems * 4)
     dw = M.DeviceBuffer(max(plan.info.work_bytes, 256))
