@@ -34,6 +34,46 @@ hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s)
     return hipGetLastError();
 }
 
+// 2D device copy (rows of `width` bytes): a kernel rather than
+// hipMemcpy2DAsync, whose device-to-device path runs on a copy engine at a
+// fraction of HBM bandwidth.  16-byte words when everything is 16-aligned.
+__global__ void copy2d_u4_kernel(uint4 *dst, int64_t dpitch, const uint4 *src, int64_t spitch,
+                                 uint32_t words, uint32_t height)
+{
+    const uint32_t w = blockIdx.x * 256u + threadIdx.x;
+    if (w >= words) return;
+    for (int64_t r = blockIdx.y; r < height; r += gridDim.y) dst[r * dpitch + w] = src[r * spitch + w];
+}
+
+__global__ void copy2d_u8_kernel(uint8_t *dst, int64_t dpitch, const uint8_t *src, int64_t spitch,
+                                 uint32_t width, uint32_t height)
+{
+    for (int64_t r = blockIdx.y; r < height; r += gridDim.y)
+        for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < width; i += gridDim.x * 256u)
+            dst[r * dpitch + i] = src[r * spitch + i];
+}
+
+hipError_t launch_copy2d(void *dst, int64_t dpitch, const void *src, int64_t spitch,
+                         int64_t width, uint32_t height, hipStream_t s)
+{
+    if (width <= 0 || height == 0) return hipSuccess;
+    if (width > 0xffffffffLL) return hipErrorInvalidValue;
+    const uint32_t gy = std::min<uint32_t>(height, 65535u);
+    const bool al = (((uintptr_t)dst | (uintptr_t)src | (uint64_t)dpitch | (uint64_t)spitch |
+                      (uint64_t)width) & 15) == 0;
+    if (al) {
+        const uint32_t words = (uint32_t)(width / 16);
+        hipLaunchKernelGGL(copy2d_u4_kernel, dim3((words + 255) / 256, gy), dim3(256), 0, s,
+                           (uint4 *)dst, dpitch / 16, (const uint4 *)src, spitch / 16, words,
+                           height);
+    } else {
+        const uint32_t bx = (uint32_t)std::min<int64_t>((width + 255) / 256, 1024);
+        hipLaunchKernelGGL(copy2d_u8_kernel, dim3(bx, gy), dim3(256), 0, s, (uint8_t *)dst,
+                           dpitch, (const uint8_t *)src, spitch, (uint32_t)width, height);
+    }
+    return hipGetLastError();
+}
+
 // Loads through the constant address space are scalar (s_load) when the
 // address is wave-uniform: the per-(row, DM) offsets are read this way.
 typedef __attribute__((address_space(4))) const int32_t cint32_t;
@@ -808,12 +848,18 @@ __global__ void __launch_bounds__(256)
         if (wmask & w) {
             const double wm = (double)w * mu;
             const double den = sg * sqrtw.v[wi];
+            // (num / den > thr) with den > 0 implies num > thr*den - |thr*den| 2^-40
+            // (rounding errors are below 2^-52 relative): samples at or below
+            // that bound skip the division without changing any decision
+            const double td = thr * den;
+            const double lo = td - fabs(td) * 0x1p-40;
             const int64_t last = (int64_t)n - w;
             for (int u = 1 + tid; u <= kBT; u += 256) {
                 const int64_t t = b + u;
                 if (t > last) break;
                 const double sv = buf[cur][u];
                 const double num = sv - wm;
+                if (!(num > lo)) continue;
                 const double snr = num / den;
                 if (!(snr > thr)) continue;
                 if (t > 0 && !(sv >= buf[cur][u - 1])) continue;

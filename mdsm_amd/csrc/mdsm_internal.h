@@ -189,6 +189,8 @@ struct TimedLaunch {
 };
 
 // Launchers (kernels.hip).
+hipError_t launch_copy2d(void *dst, int64_t dpitch, const void *src, int64_t spitch,
+                         int64_t width, uint32_t height, hipStream_t s);
 hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in,
                              int64_t in_stride, float *out, void *work, hipStream_t s);
 hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s);

@@ -107,6 +107,7 @@ struct Job {
     uint32_t *count = nullptr;
     double *mean = nullptr, *sigma = nullptr;
     int32_t *ncl = nullptr;
+    char *dstage = nullptr;  // device copy of the new samples [C][N]
     int status = MDSM_OK;
     uint32_t batch = 0;
 };
@@ -129,7 +130,7 @@ struct Beam {
     uint64_t next_seq = 0;
     uint64_t trans_bytes = 0;   // transient bytes per block
     uint64_t off_out = 0, off_work = 0, off_cands = 0, off_count = 0, off_mean = 0,
-             off_sigma = 0, off_ncl = 0;
+             off_sigma = 0, off_ncl = 0, off_in = 0;
     // host result staging (pinned)
     mdsm_cand *h_cands = nullptr;
     uint32_t *h_count = nullptr;
@@ -194,6 +195,8 @@ int beam_transient_layout(Beam *b, uint32_t cap)
     o = up256(o + (uint64_t)pi.ndms_total * 8);
     b->off_ncl = o;
     o = up256(o + (uint64_t)b->obs.nchans * 4);
+    b->off_in = o;
+    o = up256(o + (uint64_t)b->obs.nchans * b->N * b->esize);
     b->trans_bytes = o;
     return MDSM_OK;
 }
@@ -257,15 +260,20 @@ int launch_batch(mdsm_server *s, Batch *bt)
         Beam *b = j->beam;
         const size_t C = b->obs.nchans, M = b->M, N = b->N, es = b->esize;
         const size_t S = M + N;
-        hipError_t e = hipMemcpy2DAsync((char *)b->ring + M * es, S * es, j->staging, N * es,
-                                        N * es, C, hipMemcpyHostToDevice, st);
+        // one contiguous DMA of the block, then a kernel places it behind the
+        // overlap (the clipper when it runs, a copy otherwise)
+        hipError_t e = hipMemcpyAsync(j->dstage, j->staging, C * N * es, hipMemcpyHostToDevice, st);
         if (e != hipSuccess) return hip_fail(e, "server H2D");
         if (b->det.clip_nsigma > 0.0f && b->obs.dtype == MDSM_U8) {
             TimedLaunch tl("clip", st);
-            e = launch_clip_u8((const uint8_t *)b->ring, (int64_t)S, (uint32_t)C, (int64_t)M,
-                               (uint32_t)N, b->det.clip_nsigma, (uint8_t *)b->ring, (int64_t)S,
-                               nullptr, nullptr, j->ncl, st);
+            e = launch_clip_u8((const uint8_t *)j->dstage, (int64_t)N, (uint32_t)C, 0, (uint32_t)N,
+                               b->det.clip_nsigma, (uint8_t *)b->ring + M, (int64_t)S, nullptr,
+                               nullptr, j->ncl, st);
             if (e != hipSuccess) return hip_fail(e, "server clip");
+        } else {
+            e = launch_copy2d((char *)b->ring + M * es, (int64_t)(S * es), j->dstage,
+                              (int64_t)(N * es), (int64_t)(N * es), (uint32_t)C, st);
+            if (e != hipSuccess) return hip_fail(e, "server block placement");
         }
         for (const DecimStep &d : b->plan->decim) {
             TimedLaunch tl("decimate", st);
@@ -345,14 +353,14 @@ int launch_batch(mdsm_server *s, Batch *bt)
         const size_t C = b->obs.nchans, M = b->M, N = b->N, es = b->esize, S = M + N;
         if (M) {
             if (N >= M) {
-                e = hipMemcpy2DAsync(b->ring, S * es, (char *)b->ring + N * es, S * es, M * es, C,
-                                     hipMemcpyDeviceToDevice, st);
+                e = launch_copy2d(b->ring, (int64_t)(S * es), (char *)b->ring + N * es,
+                                  (int64_t)(S * es), (int64_t)(M * es), (uint32_t)C, st);
             } else {
-                e = hipMemcpy2DAsync(b->tail, M * es, (char *)b->ring + N * es, S * es, M * es, C,
-                                     hipMemcpyDeviceToDevice, st);
+                e = launch_copy2d(b->tail, (int64_t)(M * es), (char *)b->ring + N * es,
+                                  (int64_t)(S * es), (int64_t)(M * es), (uint32_t)C, st);
                 if (e == hipSuccess)
-                    e = hipMemcpy2DAsync(b->ring, S * es, b->tail, M * es, M * es, C,
-                                         hipMemcpyDeviceToDevice, st);
+                    e = launch_copy2d(b->ring, (int64_t)(S * es), b->tail, (int64_t)(M * es),
+                                      (int64_t)(M * es), (uint32_t)C, st);
             }
             if (e != hipSuccess) return hip_fail(e, "server overlap save");
         }
@@ -453,6 +461,7 @@ void scheduler(mdsm_server *s)
                 j->mean = (double *)(base + b->off_mean);
                 j->sigma = (double *)(base + b->off_sigma);
                 j->ncl = (int32_t *)(base + b->off_ncl);
+                j->dstage = base + b->off_in;
                 b->busy = true;
                 bt->jobs.push_back(j);
                 s->st.admitted++;
