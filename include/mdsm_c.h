@@ -278,6 +278,43 @@ int mdsm_detect_median5(const float *d_x, uint32_t ndms, uint32_t n,
                         double *d_stats, void *stream);
 
 /* ------------------------------------------------------------------ */
+/* Batches: many plans (beams) per launch, HIP-graph replay            */
+/* ------------------------------------------------------------------ */
+
+/* A batch runs up to 64 plans of any shapes, dtypes and decimations as one
+ * step: each plan's decimation kernels, then one dedispersion launch per
+ * stage index over the union of the plans' work (longest items first, a
+ * persistent work queue when the plans' tuning asks for it).  Plan i reads
+ * d_ins[i] (row stride in_strides[i]) and writes d_outs[i]; d_works[i] is
+ * its workspace (d_works or an entry NULL: the batch allocates one).  Each
+ * plan's results equal mdsm_dedisperse on that plan alone.  The plans must
+ * outlive the batch.  Replaces the per-thread launch loop of
+ * src/mdsm/dedispersion_thread.cu:215-243 (one GPU, one beam per call);
+ * the batching itself has no counterpart in the reference. */
+typedef struct mdsm_batch mdsm_batch;
+
+int mdsm_batch_create(mdsm_plan *const *plans, uint32_t nplans, mdsm_batch **out);
+void mdsm_batch_destroy(mdsm_batch *batch);
+int mdsm_batch_run(mdsm_batch *batch, const void *const *d_ins,
+                   const int64_t *in_strides, float *const *d_outs,
+                   void *const *d_works, void *stream);
+
+/* Capture one step on `stream` (non-NULL) into a HIP graph; replay it with
+ * mdsm_batch_replay.  The graph holds the buffers given here.  Changing a
+ * plan's tuning afterwards requires a new capture. */
+int mdsm_batch_capture(mdsm_batch *batch, const void *const *d_ins,
+                       const int64_t *in_strides, float *const *d_outs,
+                       void *const *d_works, void *stream);
+int mdsm_batch_replay(mdsm_batch *batch, void *stream);
+
+/* Autotune the batch as a whole (same candidates as mdsm_plan_autotune,
+ * applied to every plan); drops a captured graph. */
+int mdsm_batch_autotune(mdsm_batch *batch, const void *const *d_ins,
+                        const int64_t *in_strides, float *const *d_outs,
+                        void *const *d_works, void *stream, uint32_t reps,
+                        mdsm_tuning *best);
+
+/* ------------------------------------------------------------------ */
 /* Streaming server: resident beams, continuous batching               */
 /* ------------------------------------------------------------------ */
 /* Replaces the reference's per-GPU worker loop and barrier protocol
@@ -291,7 +328,7 @@ typedef struct mdsm_server mdsm_server;
 typedef struct {
     uint64_t hbm_budget;    /* device bytes the server may hold (0: 90% of free memory) */
     uint32_t max_inflight;  /* blocks on the device at once (0: 16) */
-    uint32_t max_batch;     /* blocks per dedispersion launch (0: 16; <= 32) */
+    uint32_t max_batch;     /* blocks per dedispersion launch (0: 16; <= 64) */
     uint32_t persistent;    /* 1: persistent workgroups pulling from a device work counter */
     uint32_t queue_depth;   /* pushed-but-unadmitted blocks per beam (0: 4) */
 } mdsm_server_config;

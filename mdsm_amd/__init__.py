@@ -141,6 +141,13 @@ def lib():
         "mdsm_timing_get": [C.c_char_p, C.POINTER(f64), C.POINTER(u64)],
         "mdsm_corner_turn": [vp, i32, u32, u32, i64, i32, vp, i32, i64, i64, vp],
         "mdsm_detrend": [vp, u32, u32, i64, u32, vp],
+        "mdsm_batch_create": [C.POINTER(vp), u32, C.POINTER(vp)],
+        "mdsm_batch_run": [vp, C.POINTER(vp), C.POINTER(i64), C.POINTER(vp), C.POINTER(vp), vp],
+        "mdsm_batch_capture": [vp, C.POINTER(vp), C.POINTER(i64), C.POINTER(vp), C.POINTER(vp),
+                               vp],
+        "mdsm_batch_replay": [vp, vp],
+        "mdsm_batch_autotune": [vp, C.POINTER(vp), C.POINTER(i64), C.POINTER(vp), C.POINTER(vp),
+                                vp, u32, C.POINTER(Tuning)],
         "mdsm_plan_set_tuning": [vp, C.POINTER(Tuning)],
         "mdsm_plan_get_tuning": [vp, C.POINTER(Tuning)],
         "mdsm_plan_autotune": [vp, vp, i64, vp, vp, vp, u32, C.POINTER(Tuning)],
@@ -165,6 +172,8 @@ def lib():
         fn.restype = C.c_int
     L.mdsm_plan_destroy.argtypes = [vp]
     L.mdsm_plan_destroy.restype = None
+    L.mdsm_batch_destroy.argtypes = [vp]
+    L.mdsm_batch_destroy.restype = None
     L.mdsm_last_error.restype = C.c_char_p
     L.mdsm_version.restype = C.c_char_p
     _lib = L
@@ -440,6 +449,54 @@ def clip_u8(d_in, in_stride, nchans, t0, n, nsigma, d_out, out_stride, d_med=Non
     g = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
     check(lib().mdsm_clip_u8(g(d_in), in_stride, nchans, t0, n, nsigma, g(d_out), out_stride,
                              g(d_med), g(d_mad), g(d_ncl), _sh(stream)))
+
+
+class Batch:
+    """Up to 64 plans run as one step (one dedispersion launch per stage)."""
+
+    def __init__(self, plans: Sequence["Plan"]):
+        self.plans = list(plans)
+        arr = (C.c_void_p * len(self.plans))(*[p.handle for p in self.plans])
+        h = C.c_void_p()
+        check(lib().mdsm_batch_create(arr, len(self.plans), C.byref(h)))
+        self._h = h.value
+
+    def _bufs(self, ins, strides, outs, works):
+        g = lambda b: (b.ptr if isinstance(b, DeviceBuffer) else b) or None
+        n = len(self.plans)
+        a_in = (C.c_void_p * n)(*[g(b) for b in ins])
+        a_st = (C.c_int64 * n)(*[int(x) for x in strides])
+        a_out = (C.c_void_p * n)(*[g(b) for b in outs])
+        a_w = (C.c_void_p * n)(*[g(b) for b in works]) if works is not None else None
+        return a_in, a_st, a_out, a_w
+
+    def run(self, ins, strides, outs, works=None, stream=None):
+        a = self._bufs(ins, strides, outs, works)
+        check(lib().mdsm_batch_run(self._h, *a, _sh(stream)))
+
+    def capture(self, ins, strides, outs, works=None, stream=None):
+        a = self._bufs(ins, strides, outs, works)
+        check(lib().mdsm_batch_capture(self._h, *a, _sh(stream)))
+
+    def replay(self, stream=None):
+        check(lib().mdsm_batch_replay(self._h, _sh(stream)))
+
+    def autotune(self, ins, strides, outs, works=None, stream=None, reps: int = 2) -> dict:
+        a = self._bufs(ins, strides, outs, works)
+        t = Tuning()
+        check(lib().mdsm_batch_autotune(self._h, *a, _sh(stream), int(reps), C.byref(t)))
+        return t.as_dict()
+
+    def close(self):
+        if self._h:
+            lib().mdsm_batch_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def corner_turn(d_in, dtype_in, nsamp, nchans, in_stride, d_out, out_stride, out_col0=0,

@@ -83,6 +83,15 @@ TimedLaunch::~TimedLaunch()
     if (t.pending.size() > 4096) t.drain_locked();
 }
 
+bool timing_swap(bool on)
+{
+    TimingState &t = timing();
+    std::lock_guard<std::mutex> g(t.mu);
+    const bool was = t.on;
+    t.on = on;
+    return was;
+}
+
 }  // namespace mdsm
 
 using namespace mdsm;

@@ -93,7 +93,7 @@ struct SegDesc {
 };
 
 // Buffers of one job (one block of one beam) in a launch.
-constexpr int kMaxJobs = 32;
+constexpr int kMaxJobs = 64;
 struct JobBufs {
     const void *in;
     float *out;
@@ -179,7 +179,9 @@ int upload_plan(mdsm_plan *p);
 int fail(int code, const std::string &msg);
 int hip_fail(hipError_t e, const char *what);
 
-// Timing hooks around launches (capi.cpp).
+// Timing hooks around launches (capi.cpp).  timing_swap sets the switch and
+// returns its previous state.
+bool timing_swap(bool on);
 struct TimedLaunch {
     TimedLaunch(const char *name, hipStream_t s);
     ~TimedLaunch();
