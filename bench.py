@@ -44,6 +44,8 @@ WORKLOADS = {
                  dtype="f16", nsubs=64,
                  passes=[dict(lowdm=64 * (1500.0 / 4096) / 2, dmstep=1500.0 / 4096,
                               sub_dmstep=64 * (1500.0 / 4096), ncalls=64, calldms=64)]),
+    # batched: 64 beams of uneven shapes, one launch per step, graph replay
+    "cfg4": dict(beams=64, seed=4004),
     # serving: 32 beams of 1024-ch uint8 blocks of 2^16 samples, 2048 DMs over 0-1000
     "cfg5": dict(nchans=1024, nsamp=1 << 16, fch1=1500.0, foff=-300.0 / 1024, tsamp=64e-6,
                  dtype="u8", ranges=[(0.0, 1000.0 / 2048, 2048, 1)], beams=32,
@@ -162,6 +164,92 @@ def serve(M, w, args):
         "server_stats": {k: int(v) for k, v in st.items()},
     }
 
+
+
+def batch_bench(M, w, args):
+    """Config 4: 64 beams of uneven shapes -- channels from {256, 1024, 4096},
+    samples uniform in [2^16, 2^19], uint8 or float16, 1-3 DM ranges with
+    256-4096 trials in total and decimation 1-8 -- run as one batch: the
+    autotuner picks the launch shape for the batch, then steady-state steps
+    replay a captured HIP graph.  Inputs are windows of two resident pools
+    (one uint8, one float16, 4096 x 2^19 each, seeded noise with pulses)."""
+    rng = np.random.default_rng(w["seed"])
+    fch1, bw, tsamp = 1500.0, 300.0, 64e-6
+    Cmax, Smax = 4096, 1 << 19
+    print("[bench] cfg4: generating the input pools", file=sys.stderr, flush=True)
+    pools = {
+        "u8": M.synth_filterbank(Cmax, Smax, "u8", seed=401, fch1=fch1, foff=-bw / Cmax,
+                                 tsamp=tsamp, mean=128.0, sigma=16.0,
+                                 pulses=[(300.0, 200_000, 8, 1.0)], nthreads=16),
+        "f16": M.synth_filterbank(Cmax, Smax, "f16", seed=402, fch1=fch1, foff=-bw / Cmax,
+                                  tsamp=tsamp, gain=(0.5, 2.0),
+                                  pulses=[(300.0, 200_000, 8, 0.5)], nthreads=16),
+    }
+    d_pool = {k: M.DeviceBuffer.from_host(v) for k, v in pools.items()}
+    del pools
+    plans, beams = [], []
+    while len(plans) < w["beams"]:
+        C = int(rng.choice([256, 1024, 4096]))
+        S = int(rng.integers(1 << 16, (1 << 19) + 1))
+        dt = "u8" if rng.random() < 0.5 else "f16"
+        total = int(rng.integers(256, 4097))
+        nr = int(rng.integers(1, 4))
+        cuts = np.sort(rng.choice(np.arange(1, total), nr - 1, replace=False)) if nr > 1 else []
+        counts = np.diff(np.concatenate([[0], cuts, [total]])).astype(int)
+        ks = np.sort(rng.integers(1, 9, nr))
+        ranges, lo = [], 0.0
+        for nd, k in zip(counts, ks):
+            step = 0.1 * float(k)
+            ranges.append((lo, step, int(nd), int(k)))
+            lo += nd * step
+        try:
+            p = M.Plan.brute(C, fch1, -bw / C, tsamp, S, dt, ranges)
+        except M.MdsmError:
+            continue                      # block too short for that plan's delay: redraw
+        plans.append(p)
+        beams.append(dict(C=C, S=S, dtype=dt, ranges=ranges))
+    units = sum(p.info.dm_chan_samples for p in plans)
+    out_bytes = sum(p.info.out_elems * 4 for p in plans)
+    print(f"[bench] cfg4: {len(plans)} beams, {units:.3e} DM*chan*samples per step, "
+          f"outputs {out_bytes / 1e9:.1f} GB", file=sys.stderr, flush=True)
+    ins = [d_pool[b["dtype"]] for b in beams]
+    strides = [Smax] * len(beams)
+    outs = [M.DeviceBuffer(p.info.out_elems * 4) for p in plans]
+    batch = M.Batch(plans)
+    s = M.Stream()
+    t0 = time.perf_counter()
+    tuning = batch.autotune(ins, strides, outs, stream=s, reps=1)
+    tune_s = time.perf_counter() - t0
+    print(f"[bench] cfg4: autotuned {tuning} in {tune_s:.1f} s", file=sys.stderr, flush=True)
+    batch.run(ins, strides, outs, stream=s)          # eager step, timed
+    s.synchronize()
+    t0 = time.perf_counter()
+    batch.run(ins, strides, outs, stream=s)
+    s.synchronize()
+    eager_s = time.perf_counter() - t0
+    batch.capture(ins, strides, outs, stream=s)
+    for _ in range(args.warmup):
+        batch.replay(s)
+    s.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        batch.replay(s)
+    s.synchronize()
+    wall = time.perf_counter() - t0
+    n_u8 = sum(1 for b in beams if b["dtype"] == "u8")
+    return {
+        "metric": "DM*chan*samples/s (batched, graph replay)",
+        "value": units * args.steps / wall, "unit": "DM*chan*samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "u8/f16->f32", "data": "synthetic (seeded pools with pulses)",
+        "config": {"workload": f"cfg4: {len(plans)} beams (uint8 {n_u8}, float16 "
+                   f"{len(plans) - n_u8}), C in {{256,1024,4096}}, S in [2^16,2^19], "
+                   "1-3 ranges, 256-4096 trials, k 1-8", "seed": w["seed"],
+                   "dm_chan_samples_per_step": units, "output_bytes": out_bytes},
+        "tuning": tuning, "autotune_seconds": tune_s,
+        "eager_step_ms": eager_s * 1e3, "graph_step_ms": wall * 1e3 / args.steps,
+    }
 
 
 def make_plan(M, w):
@@ -291,6 +379,10 @@ def main():
         sys.exit(2)
     import mdsm_amd as M
     w = WORKLOADS[args.workload]
+    if args.workload == "cfg4":
+        print("[bench] cfg4 batched", file=sys.stderr, flush=True)
+        print(json.dumps(batch_bench(M, w, args)), flush=True)
+        return
     if args.workload == "cfg5":
         print("[bench] cfg5 serving", file=sys.stderr, flush=True)
         print(json.dumps(serve(M, w, args)), flush=True)

@@ -152,8 +152,8 @@ int mdsm_plan_dm_values(const mdsm_plan *plan, float *host_out);
  * shapes are compile-time constants (src/mdsm/cache_brute_force.h:4-6,
  * src/mdsm/dedispersion_thread.cu:69-79). */
 typedef struct {
-    uint32_t chunk_rows;  /* channels staged per LDS chunk: integer path 2..8,
-                             float path 1..16; 0 = as many as lds_budget holds */
+    uint32_t chunk_rows;  /* channels per double-buffered LDS chunk, 2..8;
+                             0 = 6, fewer where a window is too wide */
     uint32_t lds_budget;  /* LDS bytes per workgroup for staged channels;
                              0 = 65536 (the most allowed) */
     uint32_t persistent;  /* 1: persistent workgroups over a device work queue */

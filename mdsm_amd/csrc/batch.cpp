@@ -284,14 +284,7 @@ extern "C" int mdsm_batch_autotune(mdsm_batch *b, const void *const *d_ins,
     if ((rc = ensure_work(b, d_works))) return rc;
     drop_graph(b);  // a captured graph holds the old launch shapes
     hipStream_t s = (hipStream_t)stream;
-    bool any_int = false, any_float = false;
-    for (mdsm_plan *p : b->plans)
-        for (const BlockDesc &bd : p->blocks) (int_path(bd.dtype) ? any_int : any_float) = true;
-    std::vector<uint32_t> rows;
-    if (any_int) rows.insert(rows.end(), {2, 3, 4, 6, 8});
-    if (any_float) rows.insert(rows.end(), {4, 8, 12, 16});
-    std::sort(rows.begin(), rows.end());
-    rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+    const std::vector<uint32_t> rows = {2, 3, 4, 6, 8};
     std::vector<mdsm_tuning> saved;
     for (mdsm_plan *p : b->plans) saved.push_back(p->tune);
     auto set_all = [&](const mdsm_tuning &t) -> int {

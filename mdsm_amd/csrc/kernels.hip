@@ -112,6 +112,7 @@ struct DedispArgs {
     int32_t seg_begin;
     int32_t nseg;
     uint32_t *counter;  // persistent mode when non-NULL
+    float one;          // 1.0f (see dedisperse_lds_t)
     JobBufs jobs[kMaxJobs];
 };
 
@@ -125,46 +126,19 @@ struct ItemCtx {
     int64_t in_stride;
 };
 
-template <int DT>
-__device__ __forceinline__ float load_sample(const void *base, int64_t i)
-{
-    if constexpr (DT == DT_U8) {
-        return (float)((const uint8_t *)base)[i];
-    } else if constexpr (DT == DT_U16) {
-        return (float)((const uint16_t *)base)[i];
-    } else if constexpr (DT == DT_F16) {
-        return (float)((const _Float16 *)base)[i];
-    } else {
-        return ((const float *)base)[i];
-    }
-}
-
-// Stage rows [c0, c0+cc) of the block into LDS: wave w fills rows w, w+4, ...
-// Row i holds columns abase[c0+i] + t0 + [0, W) (zero beyond in_len).
-template <int DT>
-__device__ __forceinline__ void load_tile(float *tile, const void *src, int64_t in_off,
-                                          int64_t row0, int64_t rs, const int32_t *ab, int t0,
-                                          int W, int cc, int in_len, int wave, int lane)
-{
-    for (int i = wave; i < cc; i += kWaves) {
-        const int64_t rowbase = in_off + (row0 + i) * rs;
-        const int col0 = ab[i] + t0;
-        float *dst = tile + i * W;
-        for (int e = lane; e < W; e += 64) {
-            const int col = col0 + e;
-            dst[e] = col < in_len ? load_sample<DT>(src, rowbase + col) : 0.0f;
-        }
-    }
-}
-
-// ---- integer path (uint8 / uint16 blocks) --------------------------------
-// Row i of a chunk holds 4 phases x K pairs (K = W/4) of 8 bytes; pair k of
-// phase s packs samples u = 4k+s+{0,1,2,3} as uint16 lanes:
-//   .x = x[u] | x[u+2] << 16,   .y = x[u+1] | x[u+3] << 16.
-// A DM offset o then reads one aligned pair: phase o&3, index (o>>2) + lane,
-// giving the lane's 4 consecutive samples; two channels accumulate with one
-// v_add3_u32 per half.  Chunks hold <= 16 rows of values <= 4095, so the
-// uint16 lanes cannot overflow before the per-chunk flush to float32.
+// ---- LDS layouts ------------------------------------------------------------
+// Every path stages a chunk of channels into LDS in four phases, so that any
+// per-(channel, DM) offset o is one aligned read per lane that returns the
+// lane's 4 consecutive samples (phase o&3, entry (o>>2) + lane):
+//  * uint8 / uint16 / float16: entry k of phase s packs samples u = 4k+s+{0..3}
+//    as 16-bit lanes, .x = x[u] | x[u+2] << 16, .y = x[u+1] | x[u+3] << 16
+//    (8 bytes, ds_read_b64);
+//  * float32: entry k of phase s holds x[u..u+3] (16 bytes, ds_read_b128).
+// Integer samples are summed two channels at a time with one v_add3_u32 per
+// half into 16-bit lanes, flushed to float32 before they can overflow;
+// float16 samples are converted and added in one v_fma_mix_f32 each (x*1+acc,
+// a single rounding: the same value as converting and adding); float32
+// samples are added directly.  Channels are summed in ascending order.
 __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
 {
     return __builtin_amdgcn_perm(hi, lo, sel);
@@ -175,23 +149,35 @@ __device__ __forceinline__ uint32_t load_u(const void *base, int64_t i)
 {
     if constexpr (DT == DT_U8)
         return ((const uint8_t *)base)[i];
+    else if constexpr (DT == DT_F32)
+        return ((const uint32_t *)base)[i];
     else
         return ((const uint16_t *)base)[i];
 }
 
-// Global fetch of 8 consecutive samples (one LDS pair column) into raw
-// registers: uint8 -> 2 dwords of bytes, uint16 -> 4 dwords of halves.
+#ifndef MDSM_F32_2PH
+#define MDSM_F32_2PH 1
+#endif
+// float32 layout: MDSM_F32_2PH=1 two phases of float2 entries (a lane's 4
+// samples are 2L, 2L+1, 128+2L, 129+2L: one ds_read2st64_b64); 0: four
+// phases of float4 entries (4 consecutive samples, ds_read_b128).
+constexpr bool kF32Two = MDSM_F32_2PH != 0;
+
 template <int DT>
-struct RawWords {
-    static constexpr int n = DT == DT_U8 ? 2 : 4;
+struct Layout {
+    static constexpr int es = DT == DT_U8 ? 1 : (DT == DT_F32 ? 4 : 2);  // bytes per sample
+    static constexpr int raw = DT == DT_U8 ? 2 : (DT == DT_F32 ? 8 : 4);  // dwords per 8 samples
+    static constexpr int eb = (DT == DT_F32 && !kF32Two) ? 16 : 8;         // bytes per LDS entry
 };
 
+// Global fetch of 8 consecutive samples (one LDS entry column) into raw
+// registers.
 template <int DT>
 __device__ __forceinline__ void fetch8(const void *src, int64_t rowbase, int col, int in_len,
                                        bool aligned, uint32_t *raw)
 {
-    constexpr int n = RawWords<DT>::n;
-    constexpr int es = DT == DT_U8 ? 1 : 2;
+    constexpr int n = Layout<DT>::raw;
+    constexpr int es = Layout<DT>::es;
     if (aligned && col + 8 <= in_len) {
         const uint32_t *p = (const uint32_t *)((const char *)src + (rowbase + col) * es);
 #pragma unroll
@@ -204,51 +190,69 @@ __device__ __forceinline__ void fetch8(const void *src, int64_t rowbase, int col
             const uint32_t v = (col + j < in_len) ? load_u<DT>(src, rowbase + col + j) : 0u;
             if constexpr (DT == DT_U8)
                 raw[j >> 2] |= v << (8 * (j & 3));
+            else if constexpr (DT == DT_F32)
+                raw[j] = v;
             else
                 raw[j >> 1] |= v << (16 * (j & 1));
         }
     }
 }
 
-// Write the four phases of one pair column k from its 8 raw samples.
+// Write the four phases of entry column k from its 8 raw samples.
 template <int DT>
-__device__ __forceinline__ void store_phases(uint2 *row, int K, int k, const uint32_t *raw)
+__device__ __forceinline__ void store_phases(char *row, int K, int k, const uint32_t *raw)
 {
-    uint32_t d0[4], d1[4];
-    if constexpr (DT == DT_U8) {
-        const uint32_t a = raw[0], b = raw[1];  // samples 0-3, 4-7
+    if constexpr (DT == DT_F32 && kF32Two) {
+        // group k covers samples 4k..4k+3: entries 2k, 2k+1 of both phases
+        const int K2 = 2 * K;
+        uint2 *r2 = (uint2 *)row;
+        r2[2 * k] = make_uint2(raw[0], raw[1]);
+        r2[2 * k + 1] = make_uint2(raw[2], raw[3]);
+        r2[K2 + 2 * k] = make_uint2(raw[1], raw[2]);
+        r2[K2 + 2 * k + 1] = make_uint2(raw[3], raw[4]);
+        return;
+    } else if constexpr (DT == DT_F32) {
 #pragma unroll
-        for (int ph = 0; ph < 4; ++ph) {
-            d0[ph] = perm(b, a, 0x0c000c00u | (uint32_t)ph | ((uint32_t)(ph + 2) << 16));
-            d1[ph] = perm(b, a, 0x0c000c00u | (uint32_t)(ph + 1) | ((uint32_t)(ph + 3) << 16));
-        }
+        for (int ph = 0; ph < 4; ++ph)
+            ((uint4 *)row)[ph * K + k] = make_uint4(raw[ph], raw[ph + 1], raw[ph + 2], raw[ph + 3]);
+        return;
     } else {
-        const uint32_t h0 = raw[0], h1 = raw[1], h2 = raw[2], h3 = raw[3];
-        const uint32_t P0 = perm(h1, h0, 0x05040100u), Q0 = perm(h1, h0, 0x07060302u);
-        const uint32_t P1 = perm(h2, h1, 0x05040100u), Q1 = perm(h2, h1, 0x07060302u);
-        const uint32_t P2 = perm(h3, h2, 0x05040100u);
-        d0[0] = P0; d1[0] = Q0;
-        d0[1] = Q0; d1[1] = P1;
-        d0[2] = P1; d1[2] = Q1;
-        d0[3] = Q1; d1[3] = P2;
-    }
+        uint32_t d0[4], d1[4];
+        if constexpr (DT == DT_U8) {
+            const uint32_t a = raw[0], b = raw[1];  // samples 0-3, 4-7
 #pragma unroll
-    for (int ph = 0; ph < 4; ++ph) row[ph * K + k] = make_uint2(d0[ph], d1[ph]);
+            for (int ph = 0; ph < 4; ++ph) {
+                d0[ph] = perm(b, a, 0x0c000c00u | (uint32_t)ph | ((uint32_t)(ph + 2) << 16));
+                d1[ph] = perm(b, a, 0x0c000c00u | (uint32_t)(ph + 1) | ((uint32_t)(ph + 3) << 16));
+            }
+        } else {
+            const uint32_t h0 = raw[0], h1 = raw[1], h2 = raw[2], h3 = raw[3];
+            const uint32_t P0 = perm(h1, h0, 0x05040100u), Q0 = perm(h1, h0, 0x07060302u);
+            const uint32_t P1 = perm(h2, h1, 0x05040100u), Q1 = perm(h2, h1, 0x07060302u);
+            const uint32_t P2 = perm(h3, h2, 0x05040100u);
+            d0[0] = P0; d1[0] = Q0;
+            d0[1] = Q0; d1[1] = P1;
+            d0[2] = P1; d1[2] = Q1;
+            d0[3] = Q1; d1[3] = P2;
+        }
+#pragma unroll
+        for (int ph = 0; ph < 4; ++ph) ((uint2 *)row)[ph * K + k] = make_uint2(d0[ph], d1[ph]);
+    }
 }
 
 // Register prefetch of one chunk: wave w owns rows w, w+kWaves, ... of the
-// chunk (<= 8 rows), each up to 4 pair columns per lane (K <= 256).
+// chunk (<= 8 rows), each up to 4 entry columns per lane (K <= 256).
 constexpr int kPfRows = (8 + kWaves - 1) / kWaves, kPfIters = 4;
 
 template <int DT>
-using PreRegs = uint32_t[kPfRows][kPfIters][RawWords<DT>::n];
+using PreRegs = uint32_t[kPfRows][kPfIters][Layout<DT>::raw];
 
 template <int DT>
 __device__ __forceinline__ void fetch_chunk(PreRegs<DT> &pre, const void *src, int64_t in_off,
                                             int64_t row0, int64_t rs, const int32_t *ab, int t0,
                                             int K, int cc, int in_len, int wave, int lane)
 {
-    constexpr int es = DT == DT_U8 ? 1 : 2;
+    constexpr int es = Layout<DT>::es;
 #pragma unroll
     for (int rr = 0; rr < kPfRows; ++rr) {
         const int i = wave + kWaves * rr;
@@ -266,14 +270,14 @@ __device__ __forceinline__ void fetch_chunk(PreRegs<DT> &pre, const void *src, i
 }
 
 template <int DT>
-__device__ __forceinline__ void store_chunk(const PreRegs<DT> &pre, uint2 *buf, int K, int cc,
-                                            int wave, int lane)
+__device__ __forceinline__ void store_chunk(const PreRegs<DT> &pre, char *buf, int rb, int K,
+                                            int cc, int wave, int lane)
 {
 #pragma unroll
     for (int rr = 0; rr < kPfRows; ++rr) {
         const int i = wave + kWaves * rr;
         if (i >= cc) break;
-        uint2 *row = buf + (size_t)i * 4 * K;
+        char *row = buf + (size_t)i * rb;
 #pragma unroll
         for (int it = 0; it < kPfIters; ++it) {
             const int k = lane + 64 * it;
@@ -283,91 +287,134 @@ __device__ __forceinline__ void store_chunk(const PreRegs<DT> &pre, uint2 *buf, 
     }
 }
 
-// Integer path: chunks of ccr <= 8 channels are double-buffered in LDS; the
-// next chunk's global loads are in flight (in registers) while the current
-// chunk is summed.  Each wave reads its kRPW trials' offsets of a channel
-// pair with scalar loads; one offset serves the lane's kGroups 4-sample
-// groups (256 samples apart: the same ds_read_b64 address plus an immediate
-// 512-byte step).
-template <int DT>
-__device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
-                                 int lane, uint2 *lds)
+__device__ __forceinline__ float half_lo(uint32_t v)
 {
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(v & 0xffffu));
+}
+__device__ __forceinline__ float half_hi(uint32_t v)
+{
+    return (float)__builtin_bit_cast(_Float16, (uint16_t)(v >> 16));
+}
+
+// Chunks of ccr <= 8 channels are double-buffered in LDS; the next chunk's
+// global loads are in flight (in registers) while the current chunk is
+// summed.  Each wave reads its kRPW trials' offsets of a channel (pair) with
+// scalar loads and issues all of that channel (pair)'s LDS reads before the
+// adds.  `one` is 1.0f, passed at run time so that x*one+acc stays a fused
+// convert-and-add.
+template <int DT>
+__device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
+                                 int lane, char *lds, float one)
+{
+    constexpr bool kInt = DT == DT_U8 || DT == DT_U16;
+    constexpr int kEB = Layout<DT>::eb;
     const int W = bd.W, K = W >> 2;
     const int64_t rs = bd.src == BUF_IN ? a.in_stride : bd.row_stride;
     const void *src = bd.src == BUF_IN ? a.in : (const void *)a.work;
     const int32_t *relb = uniform_ptr(a.rel + bd.tab_off);  // [nrows][kDB] (+8 zero rows)
     const int32_t *ab = uniform_ptr(a.abase + bd.base_off);
     const int ccr = bd.ccr;
-    const int rb = 32 * K;                     // bytes per staged row
-    const size_t bufsz = (size_t)ccr * 4 * K;  // uint2 per chunk buffer
+    const int rb = 4 * K * kEB;                 // bytes per staged row
+    const size_t bufsz = (size_t)ccr * rb;      // bytes per chunk buffer
     const int nch = (bd.nrows + ccr - 1) / ccr;
 
-    float f[kRPW][kGroups][4];
-    uint32_t ae[kRPW][kGroups], ao[kRPW][kGroups];
+    float f[kRPW][4];
+    uint32_t ae[kRPW], ao[kRPW];
 #pragma unroll
-    for (int r = 0; r < kRPW; ++r)
+    for (int r = 0; r < kRPW; ++r) {
+        ae[r] = ao[r] = 0u;
 #pragma unroll
-        for (int g = 0; g < kGroups; ++g) {
-            ae[r][g] = ao[r][g] = 0u;
-#pragma unroll
-            for (int j = 0; j < 4; ++j) f[r][g][j] = 0.0f;
-        }
+        for (int j = 0; j < 4; ++j) f[r][j] = 0.0f;
+    }
     auto flush = [&]() {
-#pragma unroll
-        for (int r = 0; r < kRPW; ++r)
-#pragma unroll
-            for (int g = 0; g < kGroups; ++g) {
-                f[r][g][0] += (float)(ae[r][g] & 0xffffu);
-                f[r][g][2] += (float)(ae[r][g] >> 16);
-                f[r][g][1] += (float)(ao[r][g] & 0xffffu);
-                f[r][g][3] += (float)(ao[r][g] >> 16);
-                ae[r][g] = ao[r][g] = 0u;
-            }
-    };
-    // sum rows [0, cc) of the staged chunk into the packed accumulators
-    auto sum_chunk = [&](const char *lane_base, const int32_t *grow, int cc) {
-        int i = 0;
-        for (; i + 1 < cc; i += 2) {
-            int oi[kRPW], oj[kRPW];
-            const cint32_t *ri = as_const(grow + i * kDB);
+        if constexpr (kInt) {
 #pragma unroll
             for (int r = 0; r < kRPW; ++r) {
-                oi[r] = ri[r];
-                oj[r] = ri[kDB + r];
+                f[r][0] += (float)(ae[r] & 0xffffu);
+                f[r][2] += (float)(ae[r] >> 16);
+                f[r][1] += (float)(ao[r] & 0xffffu);
+                f[r][3] += (float)(ao[r] >> 16);
+                ae[r] = ao[r] = 0u;
             }
-            const char *rowi = lane_base + i * rb;
-            const char *rowj = rowi + rb;
-            uint2 x[kRPW][kGroups], y[kRPW][kGroups];
-#pragma unroll
-            for (int r = 0; r < kRPW; ++r)
-#pragma unroll
-                for (int g = 0; g < kGroups; ++g) {
-                    x[r][g] = *(const uint2 *)(rowi + oi[r] + 512 * g);
-                    y[r][g] = *(const uint2 *)(rowj + oj[r] + 512 * g);
-                }
-#pragma unroll
-            for (int r = 0; r < kRPW; ++r)
-#pragma unroll
-                for (int g = 0; g < kGroups; ++g) {
-                    ae[r][g] = ae[r][g] + x[r][g].x + y[r][g].x;
-                    ao[r][g] = ao[r][g] + x[r][g].y + y[r][g].y;
-                }
         }
-        if (i < cc) {
-            int oi[kRPW];
-            const cint32_t *ri = as_const(grow + i * kDB);
+    };
+    auto add_entry = [&](int r, uint2 x) {  // one 16-bit entry into trial r
+        if constexpr (DT == DT_F16) {
+            f[r][0] = __builtin_fmaf(half_lo(x.x), one, f[r][0]);
+            f[r][2] = __builtin_fmaf(half_hi(x.x), one, f[r][2]);
+            f[r][1] = __builtin_fmaf(half_lo(x.y), one, f[r][1]);
+            f[r][3] = __builtin_fmaf(half_hi(x.y), one, f[r][3]);
+        } else {
+            ae[r] += x.x;
+            ao[r] += x.y;
+        }
+    };
+    // sum rows [0, cc) of the staged chunk into the accumulators
+    auto sum_chunk = [&](const char *lane_base, const int32_t *grow, int cc) {
+        if constexpr (DT == DT_F32) {
+            for (int i = 0; i < cc; ++i) {
+                int oi[kRPW];
+                const cint32_t *ri = as_const(grow + i * kDB);
 #pragma unroll
-            for (int r = 0; r < kRPW; ++r) oi[r] = ri[r];
-            const char *rowi = lane_base + i * rb;
+                for (int r = 0; r < kRPW; ++r) oi[r] = ri[r];
+                const char *rowi = lane_base + i * rb;
+                float4 x[kRPW];
 #pragma unroll
-            for (int r = 0; r < kRPW; ++r)
-#pragma unroll
-                for (int g = 0; g < kGroups; ++g) {
-                    const uint2 x = *(const uint2 *)(rowi + oi[r] + 512 * g);
-                    ae[r][g] += x.x;
-                    ao[r][g] += x.y;
+                for (int r = 0; r < kRPW; ++r) {
+                    if constexpr (kF32Two) {
+                        const float2 *q = (const float2 *)(rowi + oi[r]);
+                        const float2 lo = q[0], hi = q[64];
+                        x[r] = make_float4(lo.x, lo.y, hi.x, hi.y);
+                    } else {
+                        x[r] = *(const float4 *)(rowi + oi[r]);
+                    }
                 }
+#pragma unroll
+                for (int r = 0; r < kRPW; ++r) {
+                    f[r][0] += x[r].x;
+                    f[r][1] += x[r].y;
+                    f[r][2] += x[r].z;
+                    f[r][3] += x[r].w;
+                }
+            }
+        } else {
+            int i = 0;
+            for (; i + 1 < cc; i += 2) {
+                int oi[kRPW], oj[kRPW];
+                const cint32_t *ri = as_const(grow + i * kDB);
+#pragma unroll
+                for (int r = 0; r < kRPW; ++r) {
+                    oi[r] = ri[r];
+                    oj[r] = ri[kDB + r];
+                }
+                const char *rowi = lane_base + i * rb;
+                const char *rowj = rowi + rb;
+                uint2 x[kRPW], y[kRPW];
+#pragma unroll
+                for (int r = 0; r < kRPW; ++r) {
+                    x[r] = *(const uint2 *)(rowi + oi[r]);
+                    y[r] = *(const uint2 *)(rowj + oj[r]);
+                }
+#pragma unroll
+                for (int r = 0; r < kRPW; ++r) {
+                    if constexpr (kInt) {
+                        ae[r] = ae[r] + x[r].x + y[r].x;
+                        ao[r] = ao[r] + x[r].y + y[r].y;
+                    } else {
+                        add_entry(r, x[r]);
+                        add_entry(r, y[r]);
+                    }
+                }
+            }
+            if (i < cc) {
+                int oi[kRPW];
+                const cint32_t *ri = as_const(grow + i * kDB);
+#pragma unroll
+                for (int r = 0; r < kRPW; ++r) oi[r] = ri[r];
+                const char *rowi = lane_base + i * rb;
+#pragma unroll
+                for (int r = 0; r < kRPW; ++r) add_entry(r, *(const uint2 *)(rowi + oi[r]));
+            }
         }
     };
 
@@ -377,7 +424,7 @@ __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, 
     {
         const int cc0 = min(ccr, bd.nrows);
         fetch_chunk<DT>(pre, src, bd.in_off, bd.row0, rs, ab, t0, K, cc0, bd.in_len, wave, lane);
-        store_chunk<DT>(pre, lds, K, cc0, wave, lane);
+        store_chunk<DT>(pre, lds, rb, K, cc0, wave, lane);
     }
     __syncthreads();
     for (int ch = 0; ch < nch; ++ch) {
@@ -389,14 +436,15 @@ __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, 
         if (more)  // next chunk's global loads overlap this chunk's compute
             fetch_chunk<DT>(pre, src, bd.in_off, bd.row0 + c1, rs, ab + c1, t0, K, cc1, bd.in_len,
                             wave, lane);
-        if (acc_rows + cc > bd.flush) {
-            flush();
-            acc_rows = 0;
+        if constexpr (kInt) {
+            if (acc_rows + cc > bd.flush) {
+                flush();
+                acc_rows = 0;
+            }
         }
-        sum_chunk((const char *)(lds + (ch & 1) * bufsz) + lane * 8,
-                  relb + (size_t)c0 * kDB + wave * kRPW, cc);
+        sum_chunk(lds + (ch & 1) * bufsz + lane * kEB, relb + (size_t)c0 * kDB + wave * kRPW, cc);
         acc_rows += cc;
-        if (more) store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, K, cc1, wave, lane);
+        if (more) store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, rb, K, cc1, wave, lane);
         __syncthreads();
     }
     flush();
@@ -407,14 +455,26 @@ __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, 
     for (int r = 0; r < kRPW; ++r) {
         const int j = wave * kRPW + r;
         if (j >= bd.ntrials) continue;
+        float v[4];
+#pragma unroll
+        for (int q = 0; q < 4; ++q) v[q] = bd.k > 1 ? __fdiv_rn(f[r][q], kf) : f[r][q];
         float *o = ob + (int64_t)j * bd.out_trial_stride;
-        const bool al = (((uintptr_t)(o + t0)) & 15) == 0;
+        if constexpr (DT == DT_F32 && kF32Two) {
+            // samples t0 + 2L + {0, 1} and t0 + 128 + 2L + {0, 1}
+            const bool al = (((uintptr_t)(o + t0)) & 7) == 0;
 #pragma unroll
-        for (int g = 0; g < kGroups; ++g) {
-            const int t = t0 + 256 * g + 4 * lane;
-            float v[4];
-#pragma unroll
-            for (int q = 0; q < 4; ++q) v[q] = bd.k > 1 ? __fdiv_rn(f[r][g][q], kf) : f[r][g][q];
+            for (int h = 0; h < 2; ++h) {
+                const int t = t0 + 128 * h + 2 * lane;
+                if (al && t + 1 < bd.nout) {
+                    *(float2 *)(o + t) = make_float2(v[2 * h], v[2 * h + 1]);
+                } else {
+                    if (t < bd.nout) o[t] = v[2 * h];
+                    if (t + 1 < bd.nout) o[t + 1] = v[2 * h + 1];
+                }
+            }
+        } else {
+            const int t = t0 + 4 * lane;
+            const bool al = (((uintptr_t)(o + t0)) & 15) == 0;
             if (al && t + 3 < bd.nout) {
                 *(float4 *)(o + t) = make_float4(v[0], v[1], v[2], v[3]);
             } else {
@@ -426,71 +486,8 @@ __device__ void dedisperse_int_t(const ItemCtx &a, const BlockDesc &bd, int t0, 
     }
 }
 
-__device__ void dedisperse_int(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
-                               int lane, uint2 *lds)
-{
-    if (bd.dtype == DT_U8)
-        dedisperse_int_t<DT_U8>(a, bd, t0, wave, lane, lds);
-    else
-        dedisperse_int_t<DT_U16>(a, bd, t0, wave, lane, lds);
-}
-
-// ---- float path (float16 / float32 blocks) -------------------------------
-__device__ void dedisperse_float(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
-                                 int lane, float *tile)
-{
-    const int W = bd.W;
-    const int64_t rs = bd.src == BUF_IN ? a.in_stride : bd.row_stride;
-    const void *src = bd.src == BUF_IN ? a.in : (const void *)a.work;
-    const int32_t *relw = uniform_ptr(a.rel + bd.tab_off + wave * kRPW);
-    const int32_t *ab = uniform_ptr(a.abase + bd.base_off);
-
-    float acc[kRPW][kPPL];
-#pragma unroll
-    for (int r = 0; r < kRPW; ++r)
-#pragma unroll
-        for (int p = 0; p < kPPL; ++p) acc[r][p] = 0.0f;
-
-    for (int c0 = 0; c0 < bd.nrows; c0 += bd.ccr) {
-        const int cc = min(bd.ccr, bd.nrows - c0);
-        __syncthreads();
-        if (bd.dtype == DT_F16)
-            load_tile<DT_F16>(tile, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, W, cc,
-                              bd.in_len, wave, lane);
-        else
-            load_tile<DT_F32>(tile, src, bd.in_off, bd.row0 + c0, rs, ab + c0, t0, W, cc,
-                              bd.in_len, wave, lane);
-        __syncthreads();
-        for (int i = 0; i < cc; ++i) {
-            const char *row = (const char *)(tile + i * W + lane);
-            const cint32_t *rl = as_const(relw + (c0 + i) * kDB);
-#pragma unroll
-            for (int r = 0; r < kRPW; ++r) {
-                const float *qp = (const float *)(row + rl[r]);
-#pragma unroll
-                for (int p = 0; p < kPPL; ++p) acc[r][p] += qp[64 * p];
-            }
-        }
-    }
-
-    float *ob = (bd.dst == BUF_OUT ? a.out : (float *)a.work) + bd.out_off;
-    const float kf = (float)bd.k;
-#pragma unroll
-    for (int r = 0; r < kRPW; ++r) {
-        const int j = wave * kRPW + r;
-        if (j < bd.ntrials) {
-            float *o = ob + (int64_t)j * bd.out_trial_stride;
-#pragma unroll
-            for (int p = 0; p < kPPL; ++p) {
-                const int t = t0 + lane + 64 * p;
-                if (t < bd.nout) o[t] = bd.k > 1 ? __fdiv_rn(acc[r][p], kf) : acc[r][p];
-            }
-        }
-    }
-}
-
 __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, int wave,
-                                                int lane, uint2 *lds)
+                                                int lane, char *lds)
 {
     // Segment lookup (wave-uniform binary search over item_start).
     int lo = a.seg_begin, hi = a.seg_begin + a.nseg - 1;
@@ -512,16 +509,12 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
     ItemCtx ctx{uniform_ptr(sg.rel), uniform_ptr(sg.abase), uniform_ptr(jb.in),
                 uniform_ptr(jb.out), uniform_ptr(jb.work), jb.in_stride};
     const int t0 = tile_i * kTile;
-#if defined(MDSM_ONLY_INT)
-    dedisperse_int(ctx, bd, t0, wave, lane, lds);
-#elif defined(MDSM_ONLY_FLOAT)
-    dedisperse_float(ctx, bd, t0, wave, lane, (float *)lds);
-#else
-    if (bd.dtype == DT_U8 || bd.dtype == DT_U16)
-        dedisperse_int(ctx, bd, t0, wave, lane, lds);
-    else
-        dedisperse_float(ctx, bd, t0, wave, lane, (float *)lds);
-#endif
+switch (bd.dtype) {
+    case DT_U8: dedisperse_lds_t<DT_U8>(ctx, bd, t0, wave, lane, lds, a.one); break;
+    case DT_U16: dedisperse_lds_t<DT_U16>(ctx, bd, t0, wave, lane, lds, a.one); break;
+    case DT_F16: dedisperse_lds_t<DT_F16>(ctx, bd, t0, wave, lane, lds, a.one); break;
+    default: dedisperse_lds_t<DT_F32>(ctx, bd, t0, wave, lane, lds, a.one); break;
+    }
 }
 
 #ifndef MDSM_DEDISP_WAVES
@@ -530,7 +523,7 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MDSM_DEDISP_WAVES)))
 dedisperse_kernel(DedispArgs a)
 {
-    extern __shared__ uint2 lds_raw[];
+    extern __shared__ __attribute__((aligned(16))) char lds_raw[];
     __shared__ int64_t sh_item;
     const int lane = threadIdx.x & 63;
     const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
@@ -571,6 +564,7 @@ hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s)
     a.seg_begin = L.seg_begin;
     a.nseg = L.nseg;
     a.counter = L.counter;
+    a.one = 1.0f;
     for (int i = 0; i < L.njobs; ++i) a.jobs[i] = L.jobs[i];
     const size_t lds = (size_t)L.lds_bytes;
     unsigned grid = (unsigned)L.items;
@@ -603,6 +597,15 @@ hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in
 // ---------------------------------------------------------------------------
 // Decimation: out[c][t] = sum_{j<k} in[c][k t + j]
 // ---------------------------------------------------------------------------
+
+template <int DT>
+__device__ __forceinline__ float load_sample(const void *base, int64_t i)
+{
+    if constexpr (DT == DT_F16)
+        return (float)((const _Float16 *)base)[i];
+    else
+        return ((const float *)base)[i];
+}
 
 template <int DTI, int DTO>
 __global__ void __launch_bounds__(256)

@@ -28,7 +28,6 @@ constexpr int kDB = kWaves * kRPW;   // DM trials per block
 constexpr int kGroups = MDSM_GROUPS; // integer path: 4-sample groups per lane, 256 apart
 constexpr int kPPL = 4 * kGroups;    // output samples per lane
 constexpr int kTile = 256 * kGroups; // output samples per tile
-constexpr int kCC = 16;              // channels per LDS chunk
 constexpr int kMaxW = 1024;          // widest LDS row (floats)
 
 // Buffers a block can read or write.
@@ -72,10 +71,24 @@ struct BlockDesc {
 // LDS bytes of one staged row: integer data (uint8/uint16) is held as four
 // phase-shifted copies of uint16 pairs (8 bytes per sample), float data as
 // float32 (4 bytes per sample).
-inline int lds_row_bytes(int dtype, int W) { return (dtype == DT_U8 || dtype == DT_U16) ? 8 * W : 4 * W; }
+// Staged LDS rows: four phases of 16-bit entries (8 bytes per 4 samples) for
+// uint8/uint16/float16; for float32 two phases of float2 entries (8 bytes
+// per 2 samples) or, with MDSM_F32_2PH=0, four phases of float4 entries.
+#ifndef MDSM_F32_2PH
+#define MDSM_F32_2PH 1
+#endif
+inline int lds_row_bytes(int dtype, int W) { return (dtype == DT_F32 && !MDSM_F32_2PH) ? 16 * W : 8 * W; }
+// Byte offset of shift o (relative to the row base) in a staged row of W samples.
+inline int32_t lds_offset(int dtype, int W, int32_t o)
+{
+    const int K = W / 4;
+    if (dtype != DT_F32) return (o & 3) * K * 8 + (o >> 2) * 8;
+    if (MDSM_F32_2PH) return (o & 1) * (2 * K) * 8 + (o >> 1) * 8;
+    return (o & 3) * K * 16 + (o >> 2) * 16;
+}
 inline bool int_path(int dtype) { return dtype == DT_U8 || dtype == DT_U16; }
 constexpr int kLdsBudget = 65536;  // bytes of LDS per workgroup
-constexpr int kIntChunkRows = 6;   // integer path: default channels per LDS chunk
+constexpr int kChunkRows = 6;      // default channels per LDS chunk (2..8)
 
 // A run of blocks tiled over the same number of output tiles; flat work
 // items of a segment are ordered tile-major (all blocks of tile 0 first).

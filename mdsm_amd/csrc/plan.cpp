@@ -141,15 +141,14 @@ int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
         b.base_off = (int32_t)p->abase.size();
         if (p->rel.size() + (size_t)s.nrows * kDB > (size_t)std::numeric_limits<int32_t>::max())
             return fail(MDSM_ERR_INVALID, "plan shift tables exceed 2^31 entries");
-        // Offsets are stored as byte offsets within the block's LDS row, so the
-        // kernel adds them to a lane address without further arithmetic.
-        const bool intpath = s.dtype == DT_U8 || s.dtype == DT_U16;
-        const int K = b.W / 4;
+        // Offsets are stored as byte offsets within the block's four-phase LDS
+        // row (phase o&3, entry o>>2), so the kernel adds them to a lane
+        // address without further arithmetic.
         for (int i = 0; i < s.nrows; ++i) {
             p->abase.push_back(base[i]);
             for (int j = 0; j < kDB; ++j) {
                 const int32_t o = j < nt ? s.sh(j0 + j, i) - base[i] : 0;
-                p->rel.push_back(intpath ? (o & 3) * K * 8 + (o >> 2) * 8 : o * 4);
+                p->rel.push_back(lds_offset(s.dtype, b.W, o));
             }
         }
         st.wmax = std::max(st.wmax, b.W);
@@ -184,10 +183,9 @@ int add_decimation(mdsm_plan *p, int k, uint64_t &work)
 }
 
 // Channels per LDS chunk of every block and each launch's dynamic LDS, from
-// the plan's tuning: the requested rows, or fewer where a block's window is
-// too wide for the LDS budget.  Integer path: two chunk buffers
-// (double-buffered), 2..8 rows, 6 by default; float path: one, 1..16 rows,
-// as many as fit by default.
+// the plan's tuning: the requested rows (2..8, 6 by default), or fewer where
+// a block's window is too wide for the LDS budget.  Chunks are
+// double-buffered.
 void apply_tuning(mdsm_plan *p)
 {
     const int budget = p->tune.lds_budget ? (int)p->tune.lds_budget : kLdsBudget;
@@ -198,17 +196,9 @@ void apply_tuning(mdsm_plan *p)
             for (int bi = sg.first_block; bi < sg.first_block + sg.nblocks; ++bi) {
                 BlockDesc &b = p->blocks[bi];
                 const int rb = lds_row_bytes(b.dtype, b.W);
-                int lds;
-                if (int_path(b.dtype)) {
-                    const int want = p->tune.chunk_rows ? (int)p->tune.chunk_rows : kIntChunkRows;
-                    b.ccr = std::max(2, std::min({8, want, budget / (2 * rb)}));
-                    lds = 2 * b.ccr * rb;
-                } else {
-                    const int want = p->tune.chunk_rows ? (int)p->tune.chunk_rows : kCC;
-                    b.ccr = std::max(1, std::min({kCC, want, budget / rb}));
-                    lds = b.ccr * rb;
-                }
-                st.lds_bytes = std::max(st.lds_bytes, lds);
+                const int want = p->tune.chunk_rows ? (int)p->tune.chunk_rows : kChunkRows;
+                b.ccr = std::max(2, std::min({8, want, budget / (2 * rb)}));
+                st.lds_bytes = std::max(st.lds_bytes, 2 * b.ccr * rb);
             }
         }
     }
@@ -739,7 +729,8 @@ extern "C" int mdsm_dedisperse(mdsm_plan *p, const void *d_in, int64_t in_stride
 extern "C" int mdsm_plan_set_tuning(mdsm_plan *p, const mdsm_tuning *t)
 {
     if (!p || !t) return fail(MDSM_ERR_INVALID, "NULL argument");
-    if (t->chunk_rows > (uint32_t)kCC) return fail(MDSM_ERR_INVALID, "chunk_rows must be <= 16");
+    if (t->chunk_rows == 1 || t->chunk_rows > 8)
+        return fail(MDSM_ERR_INVALID, "chunk_rows must be 0 or 2..8");
     if (t->lds_budget > (uint32_t)kLdsBudget)
         return fail(MDSM_ERR_INVALID, "lds_budget must be <= 65536");
     if (t->lds_budget && t->lds_budget < 4096)
@@ -775,13 +766,7 @@ extern "C" int mdsm_plan_autotune(mdsm_plan *p, const void *d_in, int64_t in_str
     if (reps < 1 || reps > 1000) return fail(MDSM_ERR_INVALID, "reps must be 1..1000");
     hipStream_t s = (hipStream_t)stream;
     std::vector<mdsm_tuning> cand;
-    bool any_int = false, any_float = false;
-    for (const BlockDesc &b : p->blocks) (int_path(b.dtype) ? any_int : any_float) = true;
-    std::vector<uint32_t> rows;
-    if (any_int) rows.insert(rows.end(), {2, 3, 4, 6, 8});
-    if (any_float) rows.insert(rows.end(), {4, 8, 12, 16});
-    std::sort(rows.begin(), rows.end());
-    rows.erase(std::unique(rows.begin(), rows.end()), rows.end());
+    const std::vector<uint32_t> rows = {2, 3, 4, 6, 8};
     for (uint32_t per = 0; per <= 1; ++per)
         for (uint32_t r : rows) {
             mdsm_tuning t{};
