@@ -186,6 +186,14 @@ int mdsm_plan_autotune(mdsm_plan *plan, const void *d_in, int64_t in_stride,
 int mdsm_dedisperse(mdsm_plan *plan, const void *d_in, int64_t in_stride,
                     float *d_out, void *d_work, void *stream);
 
+/* The same float32 brute-force result from a plain kernel in the shape of
+ * the reference's opt_dedisperse_loop (src/mdsm/dedispersion_kernel.cu:51-77):
+ * one thread per (DM, sample), channel loop of cached global gathers.  A
+ * performance baseline (float32 plans with undecimated ranges only); the
+ * sums are in the same channel order, so results equal mdsm_dedisperse. */
+int mdsm_dedisperse_plain(mdsm_plan *plan, const float *d_in, int64_t in_stride,
+                          float *d_out, void *stream);
+
 /* k-sample time decimation (sum) of a channel-major block:
  * d_out[c][t] = sum_{j<k} d_in[c][k t + j], t < nsamp/k.  uint8 input gives
  * uint16 output, float16/float32 input gives float32.
@@ -382,6 +390,17 @@ int mdsm_beam_info(mdsm_server *srv, uint32_t beam_id, uint32_t *overlap,
  * MDSM_ERR_AGAIN when the beam's queue is full (back-pressure). */
 int mdsm_beam_push(mdsm_server *srv, uint32_t beam_id, const void *host, int64_t stride,
                    double arrival_s, uint64_t *seq);
+
+/* Zero-copy push: mdsm_beam_acquire lends the beam's next free pinned
+ * staging block ([nchans][N] samples of obs.dtype, row stride N); the caller
+ * fills it and queues it with mdsm_beam_commit (arrival_s as for push).
+ * MDSM_ERR_AGAIN when no block is free.  A lent block counts against the
+ * beam's queue depth until committed.  (The reference's caller writes the
+ * library-owned input buffer in place the same way,
+ * src/pelican-mdsm/modules/MdsmModule.cpp:135-151.) */
+int mdsm_beam_acquire(mdsm_server *srv, uint32_t beam_id, void **host_block);
+int mdsm_beam_commit(mdsm_server *srv, uint32_t beam_id, void *host_block,
+                     double arrival_s, uint64_t *seq);
 
 /* Next completed block.  Its candidates (sorted by dm, width, t) are copied
  * into cands[0..min(ncand,cap)).  wait_ms < 0 waits indefinitely.

@@ -139,6 +139,7 @@ def lib():
         "mdsm_timing_enable": [C.c_int],
         "mdsm_timing_reset": [],
         "mdsm_timing_get": [C.c_char_p, C.POINTER(f64), C.POINTER(u64)],
+        "mdsm_dedisperse_plain": [vp, vp, i64, vp, vp],
         "mdsm_corner_turn": [vp, i32, u32, u32, i64, i32, vp, i32, i64, i64, vp],
         "mdsm_detrend": [vp, u32, u32, i64, u32, vp],
         "mdsm_batch_create": [C.POINTER(vp), u32, C.POINTER(vp)],
@@ -393,6 +394,12 @@ class Plan:
         pout = d_out.ptr if isinstance(d_out, DeviceBuffer) else d_out
         pw = d_work.ptr if isinstance(d_work, DeviceBuffer) else d_work
         check(lib().mdsm_dedisperse(self._h, pin, int(in_stride), pout, pw, _sh(stream)))
+
+    def dedisperse_plain(self, d_in, in_stride: int, d_out, stream=None):
+        """The reference-shaped plain float32 kernel (a performance baseline)."""
+        pin = d_in.ptr if isinstance(d_in, DeviceBuffer) else d_in
+        pout = d_out.ptr if isinstance(d_out, DeviceBuffer) else d_out
+        check(lib().mdsm_dedisperse_plain(self._h, pin, int(in_stride), pout, _sh(stream)))
 
     def set_tuning(self, chunk_rows=0, lds_budget=0, persistent=0, grid=0):
         t = Tuning(chunk_rows, lds_budget, persistent, grid)

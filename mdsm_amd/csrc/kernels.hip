@@ -74,6 +74,37 @@ hipError_t launch_copy2d(void *dst, int64_t dpitch, const void *src, int64_t spi
     return hipGetLastError();
 }
 
+// Plain single-beam float32 dedispersion in the reference GPU kernel's shape
+// (src/mdsm/dedispersion_kernel.cu:51-77, launched at
+// src/mdsm/dedispersion_thread.cu:229-231): one thread per (DM, output
+// sample), a channel loop of cached global-memory gathers.  It is the
+// baseline the staged kernel is compared with, not a product path.
+__global__ void __launch_bounds__(256)
+    plain_kernel(const float *__restrict__ in, int64_t in_stride, uint32_t nchans,
+                 const int32_t *__restrict__ shift, uint32_t nout, float *__restrict__ out)
+{
+    extern __shared__ int32_t sh[];
+    const uint32_t d = blockIdx.y;
+    for (uint32_t c = threadIdx.x; c < nchans; c += 256) sh[c] = shift[(int64_t)d * nchans + c];
+    __syncthreads();
+    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= nout) return;
+    float acc = 0.0f;
+    for (uint32_t c = 0; c < nchans; ++c) acc += in[(int64_t)c * in_stride + sh[c] + t];
+    out[(int64_t)d * nout + t] = acc;
+}
+
+hipError_t launch_plain(const float *in, int64_t in_stride, uint32_t nchans,
+                        const int32_t *shift, uint32_t ndms, uint32_t nout, float *out,
+                        hipStream_t s)
+{
+    if (ndms == 0 || nout == 0) return hipSuccess;
+    hipLaunchKernelGGL(plain_kernel, dim3((nout + 255) / 256, ndms), dim3(256),
+                       (size_t)nchans * sizeof(int32_t), s, in, in_stride, nchans, shift, nout,
+                       out);
+    return hipGetLastError();
+}
+
 // Loads through the constant address space are scalar (s_load) when the
 // address is wave-uniform: the per-(row, DM) offsets are read this way.
 typedef __attribute__((address_space(4))) const int32_t cint32_t;

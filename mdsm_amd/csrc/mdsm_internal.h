@@ -147,6 +147,7 @@ struct KernelTables {
     int32_t *rel = nullptr;
     int32_t *abase = nullptr;
     uint32_t *counter = nullptr;  // persistent work-queue counters, one per stage
+    int32_t *plain_shift = nullptr;  // raw shift tables of every range, for the plain kernel
 };
 
 // A dedispersion launch over segments already resident on the device.
@@ -206,6 +207,9 @@ struct TimedLaunch {
 // Launchers (kernels.hip).
 hipError_t launch_copy2d(void *dst, int64_t dpitch, const void *src, int64_t spitch,
                          int64_t width, uint32_t height, hipStream_t s);
+hipError_t launch_plain(const float *in, int64_t in_stride, uint32_t nchans,
+                        const int32_t *shift, uint32_t ndms, uint32_t nout, float *out,
+                        hipStream_t s);
 hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in,
                              int64_t in_stride, float *out, void *work, hipStream_t s);
 hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s);

@@ -634,6 +634,7 @@ extern "C" void mdsm_plan_destroy(mdsm_plan *p)
     if (p->dev.rel) (void)hipFree(p->dev.rel);
     if (p->dev.abase) (void)hipFree(p->dev.abase);
     if (p->dev.counter) (void)hipFree(p->dev.counter);
+    if (p->dev.plain_shift) (void)hipFree(p->dev.plain_shift);
     if (p->own_work) (void)hipFree(p->own_work);
     if (sw) (void)hipSetDevice(cur);
     delete p;
@@ -818,5 +819,48 @@ extern "C" int mdsm_plan_autotune(mdsm_plan *p, const void *d_in, int64_t in_str
         g_tune_cache[p->shape_key] = best_t;
     }
     if (best) *best = best_t;
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_dedisperse_plain(mdsm_plan *p, const float *d_in, int64_t in_stride,
+                                     float *d_out, void *stream)
+{
+    if (!p || !d_in || !d_out) return fail(MDSM_ERR_INVALID, "NULL argument");
+    if (p->subband || p->obs.dtype != MDSM_F32)
+        return fail(MDSM_ERR_INVALID, "the plain kernel runs float32 brute-force plans only");
+    for (const RangeHost &rh : p->ranges)
+        if (rh.info.downsample != 1)
+            return fail(MDSM_ERR_INVALID, "the plain kernel runs undecimated ranges only");
+    if (in_stride < (int64_t)p->obs.nsamp_in)
+        return fail(MDSM_ERR_INVALID, "in_stride < nsamp_in");
+    if (p->obs.nchans > 16384) return fail(MDSM_ERR_INVALID, "plain kernel: nchans > 16384");
+    int rc = upload_plan(p);
+    if (rc) return rc;
+    hipError_t e;
+    if (!p->dev.plain_shift) {
+        size_t n = 0;
+        for (const RangeHost &rh : p->ranges) n += rh.shift.size();
+        e = hipMalloc((void **)&p->dev.plain_shift, n * sizeof(int32_t));
+        if (e != hipSuccess) {
+            p->dev.plain_shift = nullptr;
+            return hip_fail(e, "plain shift table");
+        }
+        size_t o = 0;
+        for (const RangeHost &rh : p->ranges) {
+            e = hipMemcpy(p->dev.plain_shift + o, rh.shift.data(), rh.shift.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice);
+            if (e != hipSuccess) return hip_fail(e, "plain shift table");
+            o += rh.shift.size();
+        }
+    }
+    hipStream_t s = (hipStream_t)stream;
+    size_t o = 0;
+    for (const RangeHost &rh : p->ranges) {
+        TimedLaunch tl("plain", s);
+        e = launch_plain(d_in, in_stride, p->obs.nchans, p->dev.plain_shift + o, rh.info.ndms,
+                         rh.info.nout, d_out + rh.info.out_offset, s);
+        if (e != hipSuccess) return hip_fail(e, "plain launch");
+        o += rh.shift.size();
+    }
     return MDSM_OK;
 }

@@ -125,6 +125,7 @@ struct Beam {
     std::deque<Pending> queue;
     std::vector<void *> free_staging;
     std::vector<void *> all_staging;
+    std::vector<void *> acquired;   // handed out by mdsm_beam_acquire, not yet committed
     bool busy = false;
     bool closing = false;
     uint64_t next_seq = 0;
@@ -742,8 +743,21 @@ extern "C" int mdsm_beam_push(mdsm_server *s, uint32_t id, const void *host, int
         b->free_staging.pop_back();
     }
     const size_t C = b->obs.nchans, N = b->N, es = b->esize;
-    for (size_t c = 0; c < C; ++c)
-        std::memcpy((char *)stg + c * N * es, (const char *)host + c * (size_t)stride * es, N * es);
+    auto copy_rows = [&](size_t c0, size_t c1) {
+        for (size_t c = c0; c < c1; ++c)
+            std::memcpy((char *)stg + c * N * es, (const char *)host + c * (size_t)stride * es,
+                        N * es);
+    };
+    const size_t bytes = C * N * es;
+    const size_t nt = std::min<size_t>({8, C, bytes / (4u << 20) + 1});
+    if (nt <= 1) {
+        copy_rows(0, C);
+    } else {  // a large block: copy channel ranges on several threads
+        std::vector<std::thread> th;
+        for (size_t i = 1; i < nt; ++i) th.emplace_back(copy_rows, C * i / nt, C * (i + 1) / nt);
+        copy_rows(0, C / nt);
+        for (auto &t : th) t.join();
+    }
     {
         std::lock_guard<std::mutex> g(s->mu);
         Pending p{b->next_seq++, arrival_s >= 0 ? arrival_s : mdsm_now(), stg};
@@ -844,5 +858,48 @@ extern "C" int mdsm_server_get_stats(mdsm_server *s, mdsm_server_stats *st)
     st->bytes_in_use = s->pool.used;
     st->bytes_peak = s->pool.peak;
     st->inflight = s->inflight_blocks;
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_beam_acquire(mdsm_server *s, uint32_t id, void **host_block)
+{
+    if (!s || !host_block) return fail(MDSM_ERR_INVALID, "NULL argument");
+    std::lock_guard<std::mutex> g(s->mu);
+    auto it = s->beams.find(id);
+    if (it == s->beams.end()) return fail(MDSM_ERR_INVALID, "no such beam");
+    Beam *b = it->second;
+    if (b->closing) return fail(MDSM_ERR_STATE, "beam is closing");
+    if (b->queue.size() + b->acquired.size() >= s->cfg.queue_depth || b->free_staging.empty())
+        return fail(MDSM_ERR_AGAIN, "beam queue full");
+    void *stg = b->free_staging.back();
+    b->free_staging.pop_back();
+    b->acquired.push_back(stg);
+    *host_block = stg;
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_beam_commit(mdsm_server *s, uint32_t id, void *host_block, double arrival_s,
+                                uint64_t *seq)
+{
+    if (!s || !host_block) return fail(MDSM_ERR_INVALID, "NULL argument");
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->beams.find(id);
+        if (it == s->beams.end()) return fail(MDSM_ERR_INVALID, "no such beam");
+        Beam *b = it->second;
+        auto a = std::find(b->acquired.begin(), b->acquired.end(), host_block);
+        if (a == b->acquired.end())
+            return fail(MDSM_ERR_INVALID, "buffer was not acquired from this beam");
+        b->acquired.erase(a);
+        if (b->closing) {
+            b->free_staging.push_back(host_block);
+            return fail(MDSM_ERR_STATE, "beam is closing");
+        }
+        Pending p{b->next_seq++, arrival_s >= 0 ? arrival_s : mdsm_now(), host_block};
+        b->queue.push_back(p);
+        s->st.pushed++;
+        if (seq) *seq = p.seq;
+    }
+    s->cv_work.notify_all();
     return MDSM_OK;
 }

@@ -59,6 +59,8 @@ def _sig():
         "mdsm_beam_close": [vp, C.c_uint32],
         "mdsm_beam_info": [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(PlanInfo)],
         "mdsm_beam_push": [vp, C.c_uint32, vp, C.c_int64, C.c_double, C.POINTER(C.c_uint64)],
+        "mdsm_beam_acquire": [vp, C.c_uint32, C.POINTER(vp)],
+        "mdsm_beam_commit": [vp, C.c_uint32, vp, C.c_double, C.POINTER(C.c_uint64)],
         "mdsm_server_poll": [vp, C.POINTER(Result), vp, C.c_uint32, C.c_int32],
         "mdsm_server_output": [vp, C.c_uint32, C.c_uint64, C.POINTER(vp)],
         "mdsm_server_release": [vp, C.c_uint32, C.c_uint64],
@@ -127,6 +129,24 @@ class Server:
         seq = C.c_uint64()
         check(_sig().mdsm_beam_push(self.h, b.id, block.ctypes.data, block.shape[1], arrival,
                                     C.byref(seq)))
+        return seq.value
+
+    def acquire(self, b: Beam):
+        """Lend the beam's next free pinned staging block as a [nchans][N]
+        array to fill in place, or None when none is free."""
+        ptr = C.c_void_p()
+        rc = _sig().mdsm_beam_acquire(self.h, b.id, C.byref(ptr))
+        if rc == -8:
+            return None
+        check(rc)
+        dt = np.dtype(NP_DTYPE[b.obs.dtype])
+        n = b.obs.nchans * b.obs.nsamp_in
+        buf = (C.c_char * (n * dt.itemsize)).from_address(ptr.value)
+        return np.frombuffer(buf, dt, n).reshape(b.obs.nchans, b.obs.nsamp_in)
+
+    def commit(self, b: Beam, block: np.ndarray, arrival=-1.0) -> int:
+        seq = C.c_uint64()
+        check(_sig().mdsm_beam_commit(self.h, b.id, block.ctypes.data, arrival, C.byref(seq)))
         return seq.value
 
     def try_push(self, b: Beam, block: np.ndarray, arrival=-1.0):

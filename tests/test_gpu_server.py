@@ -161,3 +161,48 @@ def test_persistent_launch_matches(gpu):
         srv.close()
     for key in outs[0]:
         assert np.array_equal(outs[0][key], outs[1][key])
+
+
+def test_zero_copy_acquire_commit_matches_push(gpu):
+    """Blocks written in place into lent staging buffers give the same
+    results as blocks pushed by copy."""
+    from mdsm_amd.server import Server
+    M = gpu
+    print("case server acquire/commit", flush=True)
+    C, N = 64, 4096
+    xs = [_stream(M, C, N, 2, 60 + i, [(35.0, 1200, 4, 4.0)]) for i in range(2)]
+    outs = []
+    for mode in ("push", "acquire"):
+        srv = Server(max_batch=4)
+        bs = [srv.open_beam(**_beam_kwargs(C, N)) for _ in xs]
+        res = {}
+        for k in range(2):
+            for b, x in zip(bs, xs):
+                blk = np.ascontiguousarray(x[:, k * N:(k + 1) * N])
+                if mode == "push":
+                    while srv.try_push(b, blk) is None:
+                        r, c = srv.poll(wait_ms=-1)
+                        res[(r.beam, r.seq)] = (srv.output(srv.beams[r.beam], r.seq), c)
+                        srv.release(srv.beams[r.beam], r.seq)
+                else:
+                    while True:
+                        buf = srv.acquire(b)
+                        if buf is not None:
+                            break
+                        r, c = srv.poll(wait_ms=-1)
+                        res[(r.beam, r.seq)] = (srv.output(srv.beams[r.beam], r.seq), c)
+                        srv.release(srv.beams[r.beam], r.seq)
+                    buf[...] = blk
+                    srv.commit(b, buf)
+        srv.drain()
+        while True:
+            r, c = srv.poll(wait_ms=0)
+            if r is None:
+                break
+            res[(r.beam, r.seq)] = (srv.output(srv.beams[r.beam], r.seq), c)
+            srv.release(srv.beams[r.beam], r.seq)
+        outs.append({(i, k): res[(b.id, k)] for i, b in enumerate(bs) for k in range(2)})
+        srv.close()
+    for key in outs[0]:
+        assert np.array_equal(outs[0][key][0], outs[1][key][0])
+        assert _key(outs[0][key][1]) == _key(outs[1][key][1])
