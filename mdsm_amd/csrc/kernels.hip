@@ -918,6 +918,100 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Register form for widths up to 64: a wave holds 256 consecutive samples of
+// a row (4 per lane, starting one before its tile) and builds S_2w(t) =
+// S_w(t) + S_w(t+w) in registers with cross-lane shifts, the same tree and
+// FP64 arithmetic as above; each wave reports the kRT tile positions whose
+// widest sum and right neighbour lie inside its 256 samples.
+constexpr int kRT = 188;
+
+__device__ __forceinline__ double shfl_down_d(double v, int k) { return __shfl_down(v, k, 64); }
+__device__ __forceinline__ double shfl_up_d(double v, int k) { return __shfl_up(v, k, 64); }
+
+__global__ void __launch_bounds__(256)
+    boxcar_reg_kernel(const float *x, uint32_t n, int64_t stride, uint32_t wmask, uint32_t maxw,
+                      double thr, SqrtW sqrtw, mdsm_cand *cands, uint32_t cap, uint32_t *count,
+                      const double *mean, const double *sigma)
+{
+    const uint32_t d = blockIdx.y;
+    const double sg = sigma[d];
+    if (!(sg > 0.0)) return;
+    const double mu = mean[d];
+    const int lane = threadIdx.x & 63;
+    const int wave = threadIdx.x >> 6;
+    const int64_t T0 = ((int64_t)blockIdx.x * 4 + wave) * kRT;
+    if (T0 >= (int64_t)n) return;  // the whole wave
+    const float *row = x + (int64_t)d * stride;
+    const int64_t b = T0 - 1;       // sample held by lane 0, slot 0
+    double s[4];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        const int64_t t = b + 4 * lane + j;
+        s[j] = (t >= 0 && t < (int64_t)n) ? (double)row[t] : 0.0;
+    }
+    for (uint32_t w = 1, wi = 0; w <= maxw; w <<= 1, ++wi) {
+        if (wmask & w) {
+            const double wm = (double)w * mu;
+            const double den = sg * sqrtw.v[wi];
+            const double td = thr * den;
+            const double lo = td - fabs(td) * 0x1p-40;  // exact pre-filter, as above
+            const int64_t last = (int64_t)n - w;
+            bool c[4];
+            bool any = false;
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const int u = 4 * lane + j;
+                const int64_t t = b + u;
+                c[j] = u >= 1 && u <= kRT && t <= last && (s[j] - wm) > lo;
+                any = any || c[j];
+            }
+            if (__any(any)) {
+                const double prev3 = shfl_up_d(s[3], 1);
+                const double next0 = shfl_down_d(s[0], 1);
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    if (!c[j]) continue;
+                    const int64_t t = b + 4 * lane + j;
+                    const double sv = s[j];
+                    const double snr = (sv - wm) / den;
+                    if (!(snr > thr)) continue;
+                    const double pv = j > 0 ? s[j > 0 ? j - 1 : 0] : prev3;
+                    const double nv = j < 3 ? s[j < 3 ? j + 1 : 3] : next0;
+                    if (t > 0 && !(sv >= pv)) continue;
+                    if (t < last && !(sv > nv)) continue;
+                    const uint32_t slot = atomicAdd(count, 1u);
+                    if (slot < cap) {
+                        mdsm_cand cd;
+                        cd.dm = d;
+                        cd.t = (uint32_t)t;
+                        cd.width = w;
+                        cd.snr = (float)snr;
+                        cands[slot] = cd;
+                    }
+                }
+            }
+        }
+        if (2 * w > maxw) break;
+        if (w == 1) {
+            const double n0 = shfl_down_d(s[0], 1);
+            s[0] += s[1];
+            s[1] += s[2];
+            s[2] += s[3];
+            s[3] += n0;
+        } else if (w == 2) {
+            const double n0 = shfl_down_d(s[0], 1), n1 = shfl_down_d(s[1], 1);
+            s[0] += s[2];
+            s[1] += s[3];
+            s[2] += n0;
+            s[3] += n1;
+        } else {
+            const int k = (int)(w >> 2);
+#pragma unroll
+            for (int j = 0; j < 4; ++j) s[j] += shfl_down_d(s[j], k);
+        }
+    }
+}
+
 hipError_t launch_boxcar(const float *x, uint32_t ndms, uint32_t n, int64_t stride, uint32_t wmask,
                          double thr, const SqrtW &sqrtw, mdsm_cand *cands, uint32_t cap,
                          uint32_t *count, double *mean, double *sigma, hipStream_t s)
@@ -929,6 +1023,12 @@ hipError_t launch_boxcar(const float *x, uint32_t ndms, uint32_t n, int64_t stri
     hipLaunchKernelGGL(row_stats_finish, dim3((ndms + 255) / 256), dim3(256), 0, s, ndms, n, mean,
                        sigma);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, count, 1u, 0u);
+    if (maxw <= 64) {
+        const uint32_t tiles = (n + kRT - 1) / kRT;
+        hipLaunchKernelGGL(boxcar_reg_kernel, dim3((tiles + 3) / 4, ndms), dim3(256), 0, s, x, n,
+                           stride, wmask, maxw, thr, sqrtw, cands, cap, count, mean, sigma);
+        return hipGetLastError();
+    }
     dim3 grid((n + kBT - 1) / kBT, ndms);
     hipLaunchKernelGGL(boxcar_kernel, grid, dim3(256), 0, s, x, n, stride, wmask, maxw, thr, sqrtw,
                        cands, cap, count, mean, sigma);
