@@ -466,7 +466,7 @@ def main():
                    "dm_chan_samples_per_step": info.dm_chan_samples},
         "roofline": {"bound": "hbm", "achieved": achieved_gbs, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved_gbs / HBM_PEAK_GBS,
-                     "traffic": tr["traffic_bytes_per_launch"] if tr else None,
+                     "traffic": tr["traffic_bytes_per_launch"] * launches_per_step if tr else None,
                      "traffic_source": os.path.relpath(prof, os.path.dirname(PROFILE_DIR))
                      if tr else None,
                      "kernel": "dedisperse_kernel", "avg_launch_ms": avg_launch_s * 1e3,

@@ -158,6 +158,9 @@ typedef struct {
                              0 = 65536 (the most allowed) */
     uint32_t persistent;  /* 1: persistent workgroups over a device work queue */
     uint32_t grid;        /* persistent workgroups; 0 = 8 per compute unit */
+    uint32_t tile_group;  /* work order: consecutive items cover this many time
+                             tiles of one DM block before the next block
+                             (0 or 1: every block of a tile, then the next tile) */
 } mdsm_tuning;
 
 int mdsm_plan_set_tuning(mdsm_plan *plan, const mdsm_tuning *t);

@@ -65,7 +65,7 @@ class PlanInfo(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("chunk_rows", C.c_uint32), ("lds_budget", C.c_uint32),
-                ("persistent", C.c_uint32), ("grid", C.c_uint32)]
+                ("persistent", C.c_uint32), ("grid", C.c_uint32), ("tile_group", C.c_uint32)]
 
     def as_dict(self) -> dict:
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
@@ -401,8 +401,8 @@ class Plan:
         pout = d_out.ptr if isinstance(d_out, DeviceBuffer) else d_out
         check(lib().mdsm_dedisperse_plain(self._h, pin, int(in_stride), pout, _sh(stream)))
 
-    def set_tuning(self, chunk_rows=0, lds_budget=0, persistent=0, grid=0):
-        t = Tuning(chunk_rows, lds_budget, persistent, grid)
+    def set_tuning(self, chunk_rows=0, lds_budget=0, persistent=0, grid=0, tile_group=0):
+        t = Tuning(chunk_rows, lds_budget, persistent, grid, tile_group)
         check(lib().mdsm_plan_set_tuning(self._h, C.byref(t)))
 
     def tuning(self) -> dict:

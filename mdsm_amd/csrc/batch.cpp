@@ -111,13 +111,15 @@ int enqueue(mdsm_batch *b, const void *const *ins, const int64_t *strides, float
         L.nseg = b->stage_nseg[k];
         L.items = b->stage_items[k];
         bool persistent = false;
-        uint32_t grid = 0;
+        uint32_t grid = 0, group = 1;
         for (size_t i = 0; i < n; ++i) {
             const mdsm_plan *p = b->plans[i];
             if (k < p->stages.size()) L.lds_bytes = std::max(L.lds_bytes, p->stages[k].lds_bytes);
             persistent = persistent || p->tune.persistent;
             grid = std::max(grid, p->tune.grid);
+            group = std::max(group, p->tune.tile_group);
         }
+        L.group = (int32_t)group;
         if (persistent) {
             L.counter = b->counters + k;
             L.grid = grid ? (int32_t)grid : 8 * std::max(1, b->cus);
@@ -283,10 +285,9 @@ extern "C" int mdsm_batch_autotune(mdsm_batch *b, const void *const *d_ins,
     if (rc) return rc;
     if ((rc = ensure_work(b, d_works))) return rc;
     drop_graph(b);  // a captured graph holds the old launch shapes
-    hipStream_t s = (hipStream_t)stream;
-    const std::vector<uint32_t> rows = {2, 3, 4, 6, 8};
     std::vector<mdsm_tuning> saved;
     for (mdsm_plan *p : b->plans) saved.push_back(p->tune);
+    hipStream_t s = (hipStream_t)stream;
     auto set_all = [&](const mdsm_tuning &t) -> int {
         for (mdsm_plan *p : b->plans) {
             int r = mdsm_plan_set_tuning(p, &t);
@@ -294,51 +295,15 @@ extern "C" int mdsm_batch_autotune(mdsm_batch *b, const void *const *d_ins,
         }
         return MDSM_OK;
     };
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipError_t e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e != hipSuccess) {
-        if (e0) (void)hipEventDestroy(e0);
-        return hip_fail(e, "hipEventCreate");
-    }
-    float best_ms = std::numeric_limits<float>::max();
-    mdsm_tuning best_t{};
-    bool found = false;
-    for (uint32_t per = 0; per <= 1 && !rc; ++per)
-        for (uint32_t r : rows) {
-            mdsm_tuning t{};
-            t.chunk_rows = r;
-            t.persistent = per;
-            if ((rc = set_all(t))) break;
-            if ((rc = enqueue(b, d_ins, in_strides, d_outs, d_works, s))) break;
-            if ((e = hipEventRecord(e0, s)) != hipSuccess) {
-                rc = hip_fail(e, "hipEventRecord");
-                break;
-            }
-            for (uint32_t i = 0; i < reps && !rc; ++i)
-                rc = enqueue(b, d_ins, in_strides, d_outs, d_works, s);
-            if (rc) break;
-            float ms = 0.0f;
-            if ((e = hipEventRecord(e1, s)) != hipSuccess ||
-                (e = hipEventSynchronize(e1)) != hipSuccess ||
-                (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) {
-                rc = hip_fail(e, "timing a candidate");
-                break;
-            }
-            if (ms < best_ms) {
-                best_ms = ms;
-                best_t = t;
-                found = true;
-            }
-        }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc || !found) {
+    mdsm_tuning found{};
+    rc = tune_search(saved[0], set_all,
+                     [&]() { return enqueue(b, d_ins, in_strides, d_outs, d_works, s); }, s, reps,
+                     &found);
+    if (rc) {
         for (size_t i = 0; i < b->plans.size(); ++i)
             (void)mdsm_plan_set_tuning(b->plans[i], &saved[i]);
-        return rc ? rc : fail(MDSM_ERR_INVALID, "no candidate launch shape");
+        return rc;
     }
-    if ((rc = set_all(best_t))) return rc;
-    if (best) *best = best_t;
+    if (best) *best = found;
     return MDSM_OK;
 }

@@ -144,6 +144,7 @@ struct DedispArgs {
     int32_t nseg;
     uint32_t *counter;  // persistent mode when non-NULL
     float one;          // 1.0f (see dedisperse_lds_t)
+    int32_t group;      // tiles per item group (>= 1)
     JobBufs jobs[kMaxJobs];
 };
 
@@ -531,10 +532,16 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
     }
     lo = __builtin_amdgcn_readfirstlane(lo);
     const SegDesc sg = *cptr(uniform_ptr(a.segs + lo));
+    // Items of a segment run in groups of G consecutive tiles; within a group
+    // the G tiles of one DM block are consecutive items, so the block's offset
+    // table is re-read from L2 by neighbouring items (G = 1: tile-major).
     const int64_t fl = f - sg.item_start;
-    const int tile_i = __builtin_amdgcn_readfirstlane((int)(fl / sg.nblocks));
-    const int bi = __builtin_amdgcn_readfirstlane(sg.first_block +
-                                                  (int)(fl - (int64_t)tile_i * sg.nblocks));
+    const int64_t gsz = (int64_t)a.group * sg.nblocks;
+    const int64_t g = fl / gsz;
+    const int64_t r = fl - g * gsz;
+    const int gt = (int)min<int64_t>(a.group, sg.ntiles - g * a.group);  // tiles in this group
+    const int tile_i = __builtin_amdgcn_readfirstlane((int)(g * a.group + r % gt));
+    const int bi = __builtin_amdgcn_readfirstlane(sg.first_block + (int)(r / gt));
     const BlockDesc bd = *cptr(uniform_ptr(sg.blocks + bi));
     const JobBufs jb = a.jobs[sg.job];
     ItemCtx ctx{uniform_ptr(sg.rel), uniform_ptr(sg.abase), uniform_ptr(jb.in),
@@ -596,6 +603,7 @@ hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s)
     a.nseg = L.nseg;
     a.counter = L.counter;
     a.one = 1.0f;
+    a.group = std::max(1, L.group);
     for (int i = 0; i < L.njobs; ++i) a.jobs[i] = L.jobs[i];
     const size_t lds = (size_t)L.lds_bytes;
     unsigned grid = (unsigned)L.items;
@@ -616,6 +624,7 @@ hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in
     L.nseg = st.seg_end - st.seg_begin;
     L.items = st.items;
     L.lds_bytes = st.lds_bytes;
+    L.group = (int32_t)p->tune.tile_group;
     if (p->tune.persistent) {
         L.counter = p->dev.counter + (&st - p->stages.data());
         L.grid = p->tune.grid ? (int32_t)p->tune.grid : 8 * std::max(1, p->cus);

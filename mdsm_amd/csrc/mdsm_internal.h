@@ -4,6 +4,7 @@
 #include <hip/hip_runtime_api.h>
 
 #include <cstdint>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -158,6 +159,7 @@ struct LaunchSpec {
     int32_t lds_bytes = 0;
     uint32_t *counter = nullptr;    // persistent mode: device work counter (zeroed here)
     int32_t grid = 0;               // persistent mode: workgroups (0 = one per item)
+    int32_t group = 1;              // tiles per item group
     int32_t njobs = 0;
     JobBufs jobs[kMaxJobs];
 };
@@ -188,6 +190,13 @@ struct mdsm_plan {
 namespace mdsm {
 
 int upload_plan(mdsm_plan *p);
+
+// Launch-shape defaults and the autotuner's search (plan.cpp): `apply` sets a
+// candidate, `step` enqueues one timed run on stream s.
+mdsm_tuning default_tuning();
+int tune_search(const mdsm_tuning &start, const std::function<int(const mdsm_tuning &)> &apply,
+                const std::function<int()> &step, hipStream_t s, uint32_t reps,
+                mdsm_tuning *best_out);
 
 // Error plumbing (capi.cpp).
 int fail(int code, const std::string &msg);

@@ -275,6 +275,77 @@ void finish_info(mdsm_plan *p, double adds, double bytes, double dcs)
 
 int upload_plan(mdsm_plan *p) { return p->uploaded ? MDSM_OK : upload(p); }
 
+mdsm_tuning default_tuning()
+{
+    mdsm_tuning t{};
+    t.chunk_rows = 0;   // kChunkRows
+    t.persistent = 1;   // persistent workgroups over the work queue
+    t.tile_group = 16;  // 16 tiles of a DM block per item group
+    return t;
+}
+
+int tune_search(const mdsm_tuning &start, const std::function<int(const mdsm_tuning &)> &apply,
+                const std::function<int()> &step, hipStream_t s, uint32_t reps,
+                mdsm_tuning *best_out)
+{
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    hipError_t e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    if (e != hipSuccess) {
+        if (e0) (void)hipEventDestroy(e0);
+        return hip_fail(e, "hipEventCreate");
+    }
+    auto time_one = [&](const mdsm_tuning &t, float *ms) -> int {
+        int rc = apply(t);
+        if (!rc) rc = step();  // warm-up
+        if (rc) return rc;
+        hipError_t r = hipEventRecord(e0, s);
+        if (r != hipSuccess) return hip_fail(r, "hipEventRecord");
+        for (uint32_t i = 0; i < reps && !rc; ++i) rc = step();
+        if (rc) return rc;
+        if ((r = hipEventRecord(e1, s)) != hipSuccess || (r = hipEventSynchronize(e1)) != hipSuccess ||
+            (r = hipEventElapsedTime(ms, e0, e1)) != hipSuccess)
+            return hip_fail(r, "timing a candidate");
+        return MDSM_OK;
+    };
+    // coordinate search: channels per chunk first, then work order
+    mdsm_tuning best = start;
+    float best_ms = std::numeric_limits<float>::max();
+    int rc = MDSM_OK;
+    for (uint32_t r : {2u, 3u, 4u, 6u, 8u}) {
+        mdsm_tuning t = start;
+        t.chunk_rows = r;
+        float ms = 0.0f;
+        if ((rc = time_one(t, &ms))) break;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    }
+    const uint32_t orders[4][2] = {{0, 1}, {1, 1}, {1, 16}, {1, 64}};
+    const mdsm_tuning base = best;
+    for (int i = 0; i < 4 && !rc; ++i) {
+        mdsm_tuning t = base;
+        t.persistent = orders[i][0];
+        t.tile_group = orders[i][1];
+        float ms = 0.0f;
+        if ((rc = time_one(t, &ms))) break;
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    if (rc) {
+        (void)apply(start);
+        return rc;
+    }
+    if ((rc = apply(best))) return rc;
+    if (best_out) *best_out = best;
+    return MDSM_OK;
+}
+
 }  // namespace mdsm
 
 using namespace mdsm;
@@ -289,6 +360,7 @@ extern "C" int mdsm_plan_create(const mdsm_obs *obs, const mdsm_range *ranges, u
     if (!ranges || nranges < 1 || nranges > 64)
         return fail(MDSM_ERR_INVALID, "need 1..64 DM ranges");
     auto *p = new mdsm_plan();
+    p->tune = default_tuning();
     p->obs = *obs;
     ShiftModel model(*obs);
     const uint32_t C = obs->nchans, S = obs->nsamp_in;
@@ -425,6 +497,7 @@ extern "C" int mdsm_plan_create_subband(const mdsm_obs *obs, uint32_t nsubs, con
         return fail(MDSM_ERR_INVALID, "need 1..64 subband passes");
     const uint32_t cps = C / nsubs;
     auto *p = new mdsm_plan();
+    p->tune = default_tuning();
     p->obs = *obs;
     p->subband = true;
     p->nsubs = nsubs;
@@ -738,6 +811,7 @@ extern "C" int mdsm_plan_set_tuning(mdsm_plan *p, const mdsm_tuning *t)
         return fail(MDSM_ERR_INVALID, "lds_budget must be 0 or >= 4096");
     if (t->persistent > 1) return fail(MDSM_ERR_INVALID, "persistent must be 0 or 1");
     if (t->grid > (1u << 20)) return fail(MDSM_ERR_INVALID, "grid must be <= 2^20");
+    if (t->tile_group > 4096) return fail(MDSM_ERR_INVALID, "tile_group must be <= 4096");
     p->tune = *t;
     apply_tuning(p);
     if (p->uploaded) {
@@ -765,60 +839,17 @@ extern "C" int mdsm_plan_autotune(mdsm_plan *p, const void *d_in, int64_t in_str
 {
     if (!p || !d_in || !d_out) return fail(MDSM_ERR_INVALID, "NULL argument");
     if (reps < 1 || reps > 1000) return fail(MDSM_ERR_INVALID, "reps must be 1..1000");
-    hipStream_t s = (hipStream_t)stream;
-    std::vector<mdsm_tuning> cand;
-    const std::vector<uint32_t> rows = {2, 3, 4, 6, 8};
-    for (uint32_t per = 0; per <= 1; ++per)
-        for (uint32_t r : rows) {
-            mdsm_tuning t{};
-            t.chunk_rows = r;
-            t.persistent = per;
-            cand.push_back(t);
-        }
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    hipError_t e = hipEventCreate(&e0);
-    if (e == hipSuccess) e = hipEventCreate(&e1);
-    if (e != hipSuccess) {
-        if (e0) (void)hipEventDestroy(e0);
-        return hip_fail(e, "hipEventCreate");
-    }
-    const mdsm_tuning saved = p->tune;
-    float best_ms = std::numeric_limits<float>::max();
-    mdsm_tuning best_t = saved;
-    int rc = MDSM_OK;
-    for (const mdsm_tuning &t : cand) {
-        if ((rc = mdsm_plan_set_tuning(p, &t))) break;
-        if ((rc = mdsm_dedisperse(p, d_in, in_stride, d_out, d_work, stream))) break;  // warm-up
-        if ((e = hipEventRecord(e0, s)) != hipSuccess) {
-            rc = hip_fail(e, "hipEventRecord");
-            break;
-        }
-        for (uint32_t i = 0; i < reps && !rc; ++i)
-            rc = mdsm_dedisperse(p, d_in, in_stride, d_out, d_work, stream);
-        if (rc) break;
-        float ms = 0.0f;
-        if ((e = hipEventRecord(e1, s)) != hipSuccess || (e = hipEventSynchronize(e1)) != hipSuccess ||
-            (e = hipEventElapsedTime(&ms, e0, e1)) != hipSuccess) {
-            rc = hip_fail(e, "timing a candidate");
-            break;
-        }
-        if (ms < best_ms) {
-            best_ms = ms;
-            best_t = t;
-        }
-    }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    if (rc) {
-        (void)mdsm_plan_set_tuning(p, &saved);
-        return rc;
-    }
-    if ((rc = mdsm_plan_set_tuning(p, &best_t))) return rc;
+    mdsm_tuning found{};
+    int rc = tune_search(
+        p->tune, [&](const mdsm_tuning &t) { return mdsm_plan_set_tuning(p, &t); },
+        [&]() { return mdsm_dedisperse(p, d_in, in_stride, d_out, d_work, stream); },
+        (hipStream_t)stream, reps, &found);
+    if (rc) return rc;
     {
         std::lock_guard<std::mutex> g(g_tune_mu);
-        g_tune_cache[p->shape_key] = best_t;
+        g_tune_cache[p->shape_key] = found;
     }
-    if (best) *best = best_t;
+    if (best) *best = found;
     return MDSM_OK;
 }
 

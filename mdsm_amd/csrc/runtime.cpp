@@ -325,6 +325,8 @@ int launch_batch(mdsm_server *s, Batch *bt)
         L.nseg = (int32_t)bt->segs_host.size() - L.seg_begin;
         L.items = items;
         L.njobs = (int32_t)bt->jobs.size();
+        for (Job *j : bt->jobs)
+            L.group = std::max<int32_t>(L.group, (int32_t)j->beam->plan->tune.tile_group);
         for (size_t ji = 0; ji < bt->jobs.size(); ++ji) {
             Job *j = bt->jobs[ji];
             L.jobs[ji] = JobBufs{j->beam->ring, j->out, j->work, (int64_t)(j->beam->M + j->beam->N)};
