@@ -223,6 +223,67 @@ int mdsm_corner_turn(const void *d_in, int32_t dtype_in, uint32_t nsamp,
                      void *d_out, int32_t dtype_out, int64_t out_stride,
                      int64_t out_col0, void *stream);
 
+/* ------------------------------------------------------------------ */
+/* SIGPROC filterbank files                                            */
+/* ------------------------------------------------------------------ */
+
+/* Header of a SIGPROC filterbank file, the fields src/mdsm/file_handler.h:6-28
+ * keeps (keywords and value types as read_header parses them,
+ * src/mdsm/file_handler.cpp:56-177). */
+typedef struct {
+    char source_name[80];
+    char rawdatafile[80];
+    int32_t machine_id, telescope_id, data_type, nchans, nbits, nifs;
+    int32_t nbeams, ibeam, barycentric, pulsarcentric, nbins, nsamples;
+    double tstart, tsamp, fch1, foff, refdm, az_start, za_start, src_raj,
+        src_dej, period;
+    int64_t npuls;
+    uint64_t header_bytes; /* offset of the first spectrum */
+    uint64_t nspectra;     /* whole spectra after the header */
+} mdsm_sigproc_header;
+
+/* Parse a file's header.  An unknown keyword is MDSM_ERR_PARSE (the
+ * reference exits).  Replaces read_header (file_handler.cpp:56-177). */
+int mdsm_sigproc_read_header(const char *path, mdsm_sigproc_header *h);
+
+/* Sequential reader of raw packed spectra (nchans*nifs*nbits/8 bytes each,
+ * time-major); *got = whole spectra read (fewer at the end of the file).
+ * The samples are unpacked on the GPU by mdsm_unpack_spectra.  Replaces the
+ * file side of read_block (file_handler.cpp:193-290) and readBinaryData
+ * (src/MDSM.cpp:100-120). */
+typedef struct mdsm_sigproc_reader mdsm_sigproc_reader;
+int mdsm_sigproc_open(const char *path, mdsm_sigproc_reader **r, mdsm_sigproc_header *h);
+int mdsm_sigproc_read(mdsm_sigproc_reader *r, uint64_t nspectra, void *host_raw,
+                      uint64_t *got);
+int mdsm_sigproc_seek(mdsm_sigproc_reader *r, uint64_t spectrum);
+void mdsm_sigproc_close(mdsm_sigproc_reader *r);
+
+/* Unpack time-major packed spectra (nbits = 1, 2, 4, 8 unsigned, 16
+ * unsigned, 32 float; sub-byte samples least significant bits first, as
+ * read_block does) into channel-major rows with the corner turn:
+ * d_out[c][out_col0 + t] = sample c of spectrum t (channel order reversed
+ * when invert != 0).  dtype_out: MDSM_U8 (nbits <= 8) or MDSM_F32.
+ * in_stride_bytes: bytes between spectra.  Replaces read_block's unpack
+ * (file_handler.cpp:193-290) and the corner turn of src/MDSM.cpp:100-120. */
+int mdsm_unpack_spectra(const void *d_raw, int32_t nbits, uint32_t nsamp,
+                        uint32_t nchans, int64_t in_stride_bytes, int32_t invert,
+                        void *d_out, int32_t dtype_out, int64_t out_stride,
+                        int64_t out_col0, void *stream);
+
+/* Dedispersed-series writer: one SIGPROC file <prefix>_<dm>.fil (float32,
+ * one channel) per DM value, header as DedispersedDataWriter writes it; each
+ * write appends, for every file, the row whose DM value is within 1e-5 of
+ * the file's.  Replaces src/pelican-mdsm/modules/DedispersedDataWriter.cpp:
+ * 17-135. */
+typedef struct mdsm_series_writer mdsm_series_writer;
+int mdsm_series_writer_open(const char *prefix, const float *dm_values, uint32_t ndm,
+                            double fch1, double foff, double tsamp, double tstart,
+                            mdsm_series_writer **w);
+int mdsm_series_writer_write(mdsm_series_writer *w, const float *host_rows,
+                             const float *row_dm_values, uint32_t nrows,
+                             uint32_t nsamp, int64_t stride);
+void mdsm_series_writer_close(mdsm_series_writer *w);
+
 /* In-place detrend and normalise of nrows rows of n floats: each segment of
  * seg_len samples (the last one may be shorter) has its least-squares line
  * a + b*x removed, x = i - L/2 + 0.5 centred on the segment, and is divided

@@ -71,6 +71,26 @@ class Tuning(C.Structure):
         return {f: int(getattr(self, f)) for f, _ in self._fields_}
 
 
+class SigprocHeader(C.Structure):
+    _fields_ = [("source_name", C.c_char * 80), ("rawdatafile", C.c_char * 80),
+                ("machine_id", C.c_int32), ("telescope_id", C.c_int32), ("data_type", C.c_int32),
+                ("nchans", C.c_int32), ("nbits", C.c_int32), ("nifs", C.c_int32),
+                ("nbeams", C.c_int32), ("ibeam", C.c_int32), ("barycentric", C.c_int32),
+                ("pulsarcentric", C.c_int32), ("nbins", C.c_int32), ("nsamples", C.c_int32),
+                ("tstart", C.c_double), ("tsamp", C.c_double), ("fch1", C.c_double),
+                ("foff", C.c_double), ("refdm", C.c_double), ("az_start", C.c_double),
+                ("za_start", C.c_double), ("src_raj", C.c_double), ("src_dej", C.c_double),
+                ("period", C.c_double), ("npuls", C.c_int64), ("header_bytes", C.c_uint64),
+                ("nspectra", C.c_uint64)]
+
+    def as_dict(self) -> dict:
+        out = {}
+        for f, _ in self._fields_:
+            v = getattr(self, f)
+            out[f] = v.decode() if isinstance(v, bytes) else v
+        return out
+
+
 class RangeInfo(C.Structure):
     _fields_ = [("lowdm", C.c_float), ("dmstep", C.c_float), ("ndms", C.c_uint32),
                 ("downsample", C.c_uint32), ("nout", C.c_uint32), ("maxshift", C.c_uint32),
@@ -140,6 +160,13 @@ def lib():
         "mdsm_timing_reset": [],
         "mdsm_timing_get": [C.c_char_p, C.POINTER(f64), C.POINTER(u64)],
         "mdsm_dedisperse_plain": [vp, vp, i64, vp, vp],
+        "mdsm_sigproc_read_header": [C.c_char_p, C.POINTER(SigprocHeader)],
+        "mdsm_sigproc_open": [C.c_char_p, C.POINTER(vp), C.POINTER(SigprocHeader)],
+        "mdsm_sigproc_read": [vp, u64, vp, C.POINTER(u64)],
+        "mdsm_sigproc_seek": [vp, u64],
+        "mdsm_unpack_spectra": [vp, i32, u32, u32, i64, i32, vp, i32, i64, i64, vp],
+        "mdsm_series_writer_open": [C.c_char_p, vp, u32, f64, f64, f64, f64, C.POINTER(vp)],
+        "mdsm_series_writer_write": [vp, vp, vp, u32, u32, i64],
         "mdsm_corner_turn": [vp, i32, u32, u32, i64, i32, vp, i32, i64, i64, vp],
         "mdsm_detrend": [vp, u32, u32, i64, u32, vp],
         "mdsm_batch_create": [C.POINTER(vp), u32, C.POINTER(vp)],
@@ -175,6 +202,9 @@ def lib():
     L.mdsm_plan_destroy.restype = None
     L.mdsm_batch_destroy.argtypes = [vp]
     L.mdsm_batch_destroy.restype = None
+    for name in ("mdsm_sigproc_close", "mdsm_series_writer_close"):
+        getattr(L, name).argtypes = [vp]
+        getattr(L, name).restype = None
     L.mdsm_last_error.restype = C.c_char_p
     L.mdsm_version.restype = C.c_char_p
     _lib = L
@@ -504,6 +534,77 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- SIGPROC files
+def sigproc_read_header(path: str) -> dict:
+    h = SigprocHeader()
+    check(lib().mdsm_sigproc_read_header(path.encode(), C.byref(h)))
+    return h.as_dict()
+
+
+class SigprocReader:
+    """Sequential reader of raw packed spectra (time-major)."""
+
+    def __init__(self, path: str):
+        h = SigprocHeader()
+        r = C.c_void_p()
+        check(lib().mdsm_sigproc_open(path.encode(), C.byref(r), C.byref(h)))
+        self._h = r.value
+        self.header = h.as_dict()
+        nifs = max(self.header["nifs"], 1)
+        self.spectrum_bytes = self.header["nchans"] * nifs * self.header["nbits"] // 8
+
+    def read(self, nspectra: int) -> np.ndarray:
+        buf = np.empty(nspectra * self.spectrum_bytes, np.uint8)
+        got = C.c_uint64()
+        check(lib().mdsm_sigproc_read(self._h, nspectra, buf.ctypes.data, C.byref(got)))
+        return buf[:got.value * self.spectrum_bytes].reshape(got.value, self.spectrum_bytes)
+
+    def seek(self, spectrum: int):
+        check(lib().mdsm_sigproc_seek(self._h, spectrum))
+
+    def close(self):
+        if self._h:
+            lib().mdsm_sigproc_close(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def unpack_spectra(d_raw, nbits, nsamp, nchans, in_stride_bytes, d_out, out_stride, out_col0=0,
+                   invert=False, dtype_out="f32", stream=None):
+    g = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
+    do = DTYPES[dtype_out] if isinstance(dtype_out, str) else int(dtype_out)
+    check(lib().mdsm_unpack_spectra(g(d_raw), int(nbits), nsamp, nchans, in_stride_bytes,
+                                    1 if invert else 0, g(d_out), do, out_stride, out_col0,
+                                    _sh(stream)))
+
+
+class SeriesWriter:
+    """One float32 SIGPROC file <prefix>_<dm>.fil per selected DM value."""
+
+    def __init__(self, prefix: str, dm_values, fch1, foff, tsamp, tstart=0.0):
+        dms = np.ascontiguousarray(dm_values, np.float32)
+        h = C.c_void_p()
+        check(lib().mdsm_series_writer_open(prefix.encode(), dms.ctypes.data, dms.size, fch1,
+                                            foff, tsamp, tstart, C.byref(h)))
+        self._h = h.value
+
+    def write(self, rows: np.ndarray, row_dm_values):
+        rows = np.ascontiguousarray(rows, np.float32)
+        dmv = np.ascontiguousarray(row_dm_values, np.float32)
+        check(lib().mdsm_series_writer_write(self._h, rows.ctypes.data, dmv.ctypes.data,
+                                             rows.shape[0], rows.shape[1], rows.shape[1]))
+
+    def close(self):
+        if self._h:
+            lib().mdsm_series_writer_close(self._h)
+            self._h = None
 
 
 def corner_turn(d_in, dtype_in, nsamp, nchans, in_stride, d_out, out_stride, out_col0=0,

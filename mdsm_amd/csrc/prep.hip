@@ -42,6 +42,47 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Packed SIGPROC spectra: sample c of a spectrum starts at bit c*NB, least
+// significant bits first within a byte (NB <= 8); 16-bit unsigned and 32-bit
+// float samples are plain arrays.
+template <int NB>
+__device__ __forceinline__ float packed_sample(const uint8_t *spec, uint32_t c)
+{
+    if constexpr (NB == 32) {
+        return ((const float *)spec)[c];
+    } else if constexpr (NB == 16) {
+        return (float)((const uint16_t *)spec)[c];
+    } else {
+        const uint32_t bit = c * NB;
+        return (float)((spec[bit >> 3] >> (bit & 7)) & ((1u << NB) - 1u));
+    }
+}
+
+template <int NB, typename TO>
+__global__ void __launch_bounds__(256)
+    unpack_kernel(const uint8_t *__restrict__ raw, int64_t in_stride, uint32_t nsamp,
+                  uint32_t nchans, int invert, TO *__restrict__ out, int64_t out_stride,
+                  int64_t col0)
+{
+    __shared__ TO tile[kCT][kCT + 1];
+    const uint32_t t0 = blockIdx.x * kCT, c0 = blockIdx.y * kCT;
+    const int lx = threadIdx.x & 63, ly = threadIdx.x >> 6;
+#pragma unroll 4
+    for (int r = ly; r < kCT; r += 4) {
+        const uint32_t t = t0 + r, c = c0 + lx;
+        if (t < nsamp && c < nchans) {
+            const uint32_t src = invert ? nchans - 1 - c : c;
+            tile[lx][r] = (TO)packed_sample<NB>(raw + (int64_t)t * in_stride, src);
+        }
+    }
+    __syncthreads();
+#pragma unroll 4
+    for (int r = ly; r < kCT; r += 4) {
+        const uint32_t c = c0 + r, t = t0 + lx;
+        if (t < nsamp && c < nchans) out[(int64_t)c * out_stride + col0 + t] = tile[r][lx];
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Detrend and normalise: per row and per segment of L samples, subtract the
 // least-squares line a + b x (x centred on the segment) and divide by the
@@ -157,5 +198,44 @@ extern "C" int mdsm_detrend(float *d_x, uint32_t nrows, uint32_t n, int64_t stri
     hipLaunchKernelGGL(detrend_kernel, grid, dim3(256), 0, s, d_x, stride, n, seg_len);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "detrend launch");
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_unpack_spectra(const void *d_raw, int32_t nbits, uint32_t nsamp,
+                                   uint32_t nchans, int64_t in_stride_bytes, int32_t invert,
+                                   void *d_out, int32_t dtype_out, int64_t out_stride,
+                                   int64_t out_col0, void *stream)
+{
+    if (!d_raw || !d_out) return fail(MDSM_ERR_INVALID, "NULL argument");
+    if (nbits != 1 && nbits != 2 && nbits != 4 && nbits != 8 && nbits != 16 && nbits != 32)
+        return fail(MDSM_ERR_INVALID, "nbits must be 1, 2, 4, 8, 16 or 32");
+    if (dtype_out != MDSM_F32 && !(dtype_out == MDSM_U8 && nbits <= 8))
+        return fail(MDSM_ERR_INVALID, "dtype_out must be MDSM_F32, or MDSM_U8 for nbits <= 8");
+    if (nsamp == 0 || nchans == 0) return MDSM_OK;
+    if (in_stride_bytes * 8 < (int64_t)nchans * nbits)
+        return fail(MDSM_ERR_INVALID, "in_stride_bytes shorter than a spectrum");
+    if (out_col0 < 0 || out_stride < out_col0 + (int64_t)nsamp)
+        return fail(MDSM_ERR_INVALID, "output rows too short for out_col0 + nsamp");
+    if (nchans > 65535u * kCT) return fail(MDSM_ERR_INVALID, "nchans too large");
+    hipStream_t s = (hipStream_t)stream;
+    TimedLaunch tl("unpack", s);
+    dim3 grid((nsamp + kCT - 1) / kCT, (nchans + kCT - 1) / kCT);
+    const int inv = invert ? 1 : 0;
+    const uint8_t *raw = (const uint8_t *)d_raw;
+#define MDSM_UNPACK(NB, TO)                                                                    \
+    hipLaunchKernelGGL((unpack_kernel<NB, TO>), grid, dim3(256), 0, s, raw, in_stride_bytes,    \
+                       nsamp, nchans, inv, (TO *)d_out, out_stride, out_col0)
+    const bool u8 = dtype_out == MDSM_U8;
+    switch (nbits) {
+    case 1: if (u8) MDSM_UNPACK(1, uint8_t); else MDSM_UNPACK(1, float); break;
+    case 2: if (u8) MDSM_UNPACK(2, uint8_t); else MDSM_UNPACK(2, float); break;
+    case 4: if (u8) MDSM_UNPACK(4, uint8_t); else MDSM_UNPACK(4, float); break;
+    case 8: if (u8) MDSM_UNPACK(8, uint8_t); else MDSM_UNPACK(8, float); break;
+    case 16: MDSM_UNPACK(16, float); break;
+    default: MDSM_UNPACK(32, float); break;
+    }
+#undef MDSM_UNPACK
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_fail(e, "unpack launch");
     return MDSM_OK;
 }

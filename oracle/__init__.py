@@ -65,6 +65,8 @@ def lib():
         L.orc_median_of_five.argtypes = [f32] * 5
         L.orc_corner_turn_f32.restype = None
         L.orc_corner_turn_f32.argtypes = [vp, i32, i32, i64, i32, vp, i64]
+        L.orc_unpack_bits.restype = None
+        L.orc_unpack_bits.argtypes = [vp, i32, i64, vp]
         L.orc_detrend_f64.restype = None
         L.orc_detrend_f64.argtypes = [vp, i32, i32, i64, i32, vp]
         _lib = L
@@ -227,12 +229,86 @@ def corner_turn_f32(x_tm, invert=False):
     return out
 
 
+def unpack_bits(raw, nbits, nvals):
+    raw = np.ascontiguousarray(np.frombuffer(bytes(raw), np.uint8))
+    out = np.empty(int(nvals), np.float32)
+    lib().orc_unpack_bits(_p(raw), int(nbits), int(nvals), _p(out))
+    return out
+
+
 def detrend_f64(x, seg_len):
     x = np.ascontiguousarray(x, np.float32)
     R, N = x.shape
     out = np.empty((R, N), np.float64)
     lib().orc_detrend_f64(_p(x), R, N, N, int(seg_len), _p(out))
     return out
+
+
+# ---------------------------------------------------------------- reference SIGPROC reader
+class RefFileHeader(C.Structure):
+    """FILE_HEADER of src/mdsm/file_handler.h:6-28 (x86-64 layout)."""
+    _fields_ = [("rawdatafile", C.c_char * 80), ("source_name", C.c_char * 80),
+                ("machine_id", C.c_int), ("telescope_id", C.c_int), ("data_type", C.c_int),
+                ("nchans", C.c_int), ("nbits", C.c_int), ("nifs", C.c_int),
+                ("scan_number", C.c_int), ("barycentric", C.c_int), ("pulsarcentric", C.c_int),
+                ("tstart", C.c_double), ("mjdobs", C.c_double), ("tsamp", C.c_double),
+                ("fch1", C.c_double), ("foff", C.c_double), ("refdm", C.c_double),
+                ("az_start", C.c_double), ("za_start", C.c_double), ("src_raj", C.c_double),
+                ("src_dej", C.c_double), ("gal_l", C.c_double), ("gal_b", C.c_double),
+                ("header_tobs", C.c_double), ("raw_fch1", C.c_double), ("raw_foff", C.c_double),
+                ("nbeams", C.c_int), ("ibeam", C.c_int), ("srcl", C.c_double),
+                ("srcb", C.c_double), ("ast0", C.c_double), ("lst0", C.c_double),
+                ("wapp_scan_number", C.c_long), ("project", C.c_char * 8),
+                ("culprits", C.c_char * 24), ("analog_power", C.c_double * 2),
+                ("frequency_table", C.c_double * 4096), ("npuls", C.c_long),
+                ("period", C.c_double), ("nbins", C.c_int), ("itmp", C.c_int),
+                ("total_bytes", C.c_int)]
+
+
+def _libc():
+    c = C.CDLL(None)
+    c.fopen.restype = C.c_void_p
+    c.fopen.argtypes = [C.c_char_p, C.c_char_p]
+    c.fclose.argtypes = [C.c_void_p]
+    c.fseek.argtypes = [C.c_void_p, C.c_long, C.c_int]
+    return c
+
+
+def ref_read_header(path):
+    """The reference's read_header on a file: (FILE_HEADER copy, total_bytes) or None
+    when oracle/_ref is not built."""
+    R = ref()
+    if R is None:
+        return None
+    fn = getattr(R, "_Z11read_headerP8_IO_FILE")
+    fn.restype = C.POINTER(RefFileHeader)
+    fn.argtypes = [C.c_void_p]
+    libc = _libc()
+    f = libc.fopen(path.encode(), b"rb")
+    try:
+        h = fn(f)
+        return RefFileHeader.from_buffer_copy(h.contents) if h else None
+    finally:
+        libc.fclose(f)
+
+
+def ref_read_block(path, offset, nbits, nvals):
+    """The reference's read_block: nvals samples from byte offset."""
+    R = ref()
+    if R is None:
+        return None
+    fn = getattr(R, "_Z10read_blockP8_IO_FILEiPfm")
+    fn.restype = C.c_ulong
+    fn.argtypes = [C.c_void_p, C.c_int, C.c_void_p, C.c_ulong]
+    libc = _libc()
+    f = libc.fopen(path.encode(), b"rb")
+    try:
+        libc.fseek(f, int(offset), 0)
+        out = np.zeros(int(nvals), np.float32)
+        n = fn(f, int(nbits), out.ctypes.data, int(nvals))
+        return out[:n]
+    finally:
+        libc.fclose(f)
 
 
 # ---------------------------------------------------------------- reference build

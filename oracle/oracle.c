@@ -577,3 +577,30 @@ void orc_detrend_f64(const float *x, int nrows, int n, int64_t stride, int seg_l
         }
     }
 }
+
+/* ------------------------------------------------------------------ */
+/* SIGPROC sample unpacking                                            */
+/* ------------------------------------------------------------------ */
+
+/* read_block's unpack (src/mdsm/file_handler.cpp:203-285): 1-bit samples
+ * least significant bit first (:211-214), 2-bit with char4ints (:18-24,
+ * :225-231), 4-bit with char2ints (low nibble first, :11-16, :245-249),
+ * 8- and 16-bit unsigned (:257-272), 32-bit float (:276-278). */
+void orc_unpack_bits(const uint8_t *raw, int nbits, int64_t nvals, float *out)
+{
+    for (int64_t i = 0; i < nvals; i++) {
+        switch (nbits) {
+        case 1: out[i] = (float)((raw[i / 8] >> (i % 8)) & 1); break;
+        case 2: out[i] = (float)((raw[i / 4] >> (2 * (i % 4))) & 3); break;
+        case 4: out[i] = (float)((raw[i / 2] >> (4 * (i % 2))) & 15); break;
+        case 8: out[i] = (float)raw[i]; break;
+        case 16: {
+            uint16_t v;
+            memcpy(&v, raw + 2 * i, 2);
+            out[i] = (float)v;
+            break;
+        }
+        default: memcpy(&out[i], raw + 4 * i, 4); break;
+        }
+    }
+}

@@ -42,6 +42,9 @@
  *       numpy least-squares line fit (polyfit) on tiny inputs; invariants
  *       of the output (zero mean, zero slope, unit RMS per segment); a pure
  *       line detrends to zero.
+ *   orc_unpack_bits
+ *       the reference's own read_block (oracle/_ref) on the same file bytes;
+ *       numpy unpackbits (little bit order) for 1-, 2- and 4-bit samples.
  */
 #ifndef MDSM_ORACLE_H
 #define MDSM_ORACLE_H
@@ -121,6 +124,10 @@ float orc_median_of_five(float a, float b, float c, float d, float e);
 /* Corner turn of a time-major float block (MdsmModule.cpp:135-151). */
 void orc_corner_turn_f32(const float *in, int nsamp, int nchans, int64_t in_stride,
                          int invert, float *out, int64_t out_stride);
+
+/* Unpack nvals SIGPROC samples of nbits (1, 2, 4, 8, 16, 32) into floats
+ * (src/mdsm/file_handler.cpp:193-290). */
+void orc_unpack_bits(const uint8_t *raw, int nbits, int64_t nvals, float *out);
 
 /* Detrend and normalise, FP64 (src/prototypes/detrend.cu:12-100). */
 void orc_detrend_f64(const float *x, int nrows, int n, int64_t stride, int seg_len,

@@ -73,3 +73,29 @@ def test_detrend_matches_oracle(gpu, n, seg):
     assert err <= 1e-6
     assert np.all(got[3, :n] == 0.0)
     assert np.array_equal(got[:, n:], x[:, n:])   # padding untouched
+
+
+@pytest.mark.parametrize("nbits,out", [(1, "u8"), (2, "u8"), (4, "u8"), (8, "u8"), (1, "f32"),
+                                       (4, "f32"), (8, "f32"), (16, "f32"), (32, "f32")])
+def test_unpack_spectra_matches_oracle(gpu, nbits, out):
+    M = gpu
+    print(f"case unpack nbits={nbits} -> {out}", flush=True)
+    rng = np.random.default_rng(20 + nbits)
+    T, Cn = 700, 192
+    sb = Cn * nbits // 8
+    stride = sb + 16                       # padded spectra
+    raw = rng.integers(0, 256, (T, stride), dtype=np.uint8)
+    if nbits == 32:                        # finite floats
+        raw[:, :sb] = np.frombuffer(rng.standard_normal(T * Cn).astype(np.float32).tobytes(),
+                                    np.uint8).reshape(T, sb)
+    exp = np.stack([oracle.unpack_bits(raw[t, :sb].tobytes(), nbits, Cn) for t in range(T)])
+    for invert in (False, True):
+        d_raw = M.DeviceBuffer.from_host(raw)
+        col0, ostride = 5, T + 9
+        npd = np.uint8 if out == "u8" else np.float32
+        d_out = M.DeviceBuffer(Cn * ostride * np.dtype(npd).itemsize)
+        M.unpack_spectra(d_raw, nbits, T, Cn, stride, d_out, ostride, col0, invert, out)
+        M.synchronize()
+        got = d_out.download(npd, (Cn, ostride))[:, col0:col0 + T].astype(np.float32)
+        ref = exp.T[::-1] if invert else exp.T
+        assert np.array_equal(got, ref), f"nbits {nbits} invert {invert}"
