@@ -146,6 +146,24 @@ int mdsm_plan_fine_table(const mdsm_plan *plan, uint32_t r, int32_t *host_out);
  * (src/pelican-mdsm/modules/MdsmModule.cpp:189,202-210). */
 int mdsm_plan_dm_values(const mdsm_plan *plan, float *host_out);
 
+/* DM plan generation: passes (DM range, step, downsampling, and with nsubs
+ * > 0 the subband DM step and calls) chosen from the smearing budget, as the
+ * reference's python/ddplan.py computes them (observation, dedisp_method,
+ * dm_steps; :6-253).  dt in seconds, f_ctr and bw in MHz, ok_smearing in ms
+ * (0: the data's own).  Brute-force passes (nsubs = 0) have ncalls = 1 and
+ * calldms = ndms and map to mdsm_range {lowdm, dmstep, ndms, binsize};
+ * subband passes feed mdsm_plan_create_subband.  *npasses is set even when
+ * cap is too small (MDSM_ERR_CAPACITY). */
+typedef struct {
+    double lodm, hidm;
+    double f_ctr, bw, dt;
+    uint32_t nchans, nsubs;
+    double ok_smearing;
+} mdsm_ddplan_params;
+
+int mdsm_ddplan(const mdsm_ddplan_params *params, mdsm_pass *passes, uint32_t cap,
+                uint32_t *npasses);
+
 /* Launch shape of a plan's dedispersion kernels: the autotuner's search
  * space.  Results do not depend on it (integer sums are exact, float sums
  * keep their channel order).  No counterpart in the reference, whose launch

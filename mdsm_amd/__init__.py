@@ -91,6 +91,12 @@ class SigprocHeader(C.Structure):
         return out
 
 
+class DDPlanParams(C.Structure):
+    _fields_ = [("lodm", C.c_double), ("hidm", C.c_double), ("f_ctr", C.c_double),
+                ("bw", C.c_double), ("dt", C.c_double), ("nchans", C.c_uint32),
+                ("nsubs", C.c_uint32), ("ok_smearing", C.c_double)]
+
+
 class RangeInfo(C.Structure):
     _fields_ = [("lowdm", C.c_float), ("dmstep", C.c_float), ("ndms", C.c_uint32),
                 ("downsample", C.c_uint32), ("nout", C.c_uint32), ("maxshift", C.c_uint32),
@@ -161,6 +167,7 @@ def lib():
         "mdsm_timing_get": [C.c_char_p, C.POINTER(f64), C.POINTER(u64)],
         "mdsm_dedisperse_plain": [vp, vp, i64, vp, vp],
         "mdsm_sigproc_read_header": [C.c_char_p, C.POINTER(SigprocHeader)],
+        "mdsm_ddplan": [C.POINTER(DDPlanParams), vp, u32, C.POINTER(u32)],
         "mdsm_sigproc_open": [C.c_char_p, C.POINTER(vp), C.POINTER(SigprocHeader)],
         "mdsm_sigproc_read": [vp, u64, vp, C.POINTER(u64)],
         "mdsm_sigproc_seek": [vp, u64],
@@ -534,6 +541,19 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+# ---------------------------------------------------------------- DM plans
+def ddplan(lodm, hidm, f_ctr, bw, nchans, dt, nsubs=0, ok_smearing=0.0) -> list:
+    """Passes (DM range, step, downsampling; subband step and calls) from the
+    smearing budget."""
+    prm = DDPlanParams(lodm, hidm, f_ctr, bw, dt, nchans, nsubs, ok_smearing)
+    n = C.c_uint32()
+    buf = (Pass * 64)()
+    check(lib().mdsm_ddplan(C.byref(prm), buf, 64, C.byref(n)))
+    return [dict(lowdm=p.lowdm, highdm=p.highdm, dmstep=p.dmstep, sub_dmstep=p.sub_dmstep,
+                 binsize=p.binsize, ndms=p.ndms, ncalls=p.ncalls, calldms=p.calldms)
+            for p in buf[:n.value]]
 
 
 # ---------------------------------------------------------------- SIGPROC files

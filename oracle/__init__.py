@@ -12,6 +12,7 @@ cpu_dedisp.c / dedispersion_output.cpp / file_handler.cpp compiled by
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -242,6 +243,78 @@ def detrend_f64(x, seg_len):
     out = np.empty((R, N), np.float64)
     lib().orc_detrend_f64(_p(x), R, N, N, int(seg_len), _p(out))
     return out
+
+
+# ---------------------------------------------------------------- DM plan (DDplan)
+_ALLOW_DDM = [0.0001, 0.0002, 0.0005, 0.001, 0.002, 0.005, 0.01, 0.02, 0.03, 0.05, 0.1, 0.2, 0.3,
+              0.5, 1.0, 2.0, 3.0, 5.0, 10.0, 20.0, 30.0, 50.0, 100.0, 200.0, 300.0, 500.0, 1000.0]
+
+
+def _dm_smear(dm, bw, f):            # python/ddplan.py:126-132
+    return 1000.0 * dm * bw / (0.0001205 * f ** 3.0)
+
+
+def _bw_smear(step, bw, f):          # :134-141
+    return _dm_smear(0.5 * step, bw, f)
+
+
+def _sub_smear(substep, nsub, bw, f):  # :151-161
+    return 0.0 if nsub == 0 else _dm_smear(0.5 * substep, bw / nsub, f)
+
+
+def _method(dt, f, bw, chw, downsamp, lodm, hidm, ddm, nsub):  # dedisp_method, :23-63
+    bws = _bw_smear(ddm, bw, f)
+    per = 0
+    if nsub:
+        per = 2
+        while True:
+            if _sub_smear((per + 2) * ddm, nsub, bw, f) > 0.8 * min(bws, 1000.0 * dt * downsamp):
+                dsub = per * ddm
+                subs = _sub_smear(dsub, nsub, bw, f)
+                break
+            per += 2
+    else:
+        dsub, subs = ddm, 0.0
+    other = math.sqrt((1000.0 * dt) ** 2.0 + (1000.0 * dt * downsamp) ** 2.0 + bws ** 2.0 +
+                      subs ** 2.0)
+    cross = min(2.0 * 0.001 * other / chw * 0.0001205 * f ** 3.0, hidm)   # :84-94, :51-52
+    n = int(math.ceil((cross - lodm) / ddm))
+    calls = 1
+    if nsub:
+        calls = int(math.ceil(n * ddm / dsub))
+        n = calls * per
+    return dict(lowdm=lodm, highdm=lodm + n * ddm, dmstep=ddm, sub_dmstep=dsub,
+                binsize=downsamp, ndms=n, ncalls=calls, calldms=per if nsub else n)
+
+
+def ddplan(lodm, hidm, f_ctr, bw, nchans, dt, nsubs=0, ok_smearing=0.0):
+    """dm_steps of python/ddplan.py:163-253, in double precision."""
+    chw = bw / nchans
+    dtms = 1000.0 * dt
+    ff = 1.2
+    ids = idm = 0
+    downsamp = 1
+    min_chan = _dm_smear(lodm + 0.5 * _ALLOW_DDM[0], chw, f_ctr)
+    min_bw = _bw_smear(_ALLOW_DDM[0], bw, f_ctr)
+    ok = max(ok_smearing, min_chan, min_bw, dtms)
+    if ff * min_chan > dtms or ok > dtms:
+        okval = ok if ok > ff * min_chan else ff * min_chan
+        while dtms * 2.0 ** (ids + 1) < okval:
+            ids += 1
+        downsamp = 2 ** ids
+    ddm = dt * downsamp * 0.0001205 * f_ctr ** 3.0 / (0.5 * bw)      # guess_DMstep, :143-149
+    while _ALLOW_DDM[idm + 1] < ff * ddm:
+        idm += 1
+    ms = [_method(dt, f_ctr, bw, chw, downsamp, lodm, hidm, _ALLOW_DDM[idm], nsubs)]
+    while ms[-1]["highdm"] < hidm:
+        ids += 1
+        downsamp = 2 ** ids
+        eff = dtms * downsamp
+        while _bw_smear(_ALLOW_DDM[idm + 1], bw, f_ctr) < ff * eff:
+            idm += 1
+        ms.append(_method(dt, f_ctr, bw, chw, downsamp, ms[-1]["highdm"], hidm,
+                          _ALLOW_DDM[idm], nsubs))
+    return ms
 
 
 # ---------------------------------------------------------------- reference SIGPROC reader
