@@ -547,7 +547,7 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
     ItemCtx ctx{uniform_ptr(sg.rel), uniform_ptr(sg.abase), uniform_ptr(jb.in),
                 uniform_ptr(jb.out), uniform_ptr(jb.work), jb.in_stride};
     const int t0 = tile_i * kTile;
-switch (bd.dtype) {
+    switch (bd.dtype) {
     case DT_U8: dedisperse_lds_t<DT_U8>(ctx, bd, t0, wave, lane, lds, a.one); break;
     case DT_U16: dedisperse_lds_t<DT_U16>(ctx, bd, t0, wave, lane, lds, a.one); break;
     case DT_F16: dedisperse_lds_t<DT_F16>(ctx, bd, t0, wave, lane, lds, a.one); break;

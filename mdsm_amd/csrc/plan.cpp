@@ -280,7 +280,7 @@ mdsm_tuning default_tuning()
     mdsm_tuning t{};
     t.chunk_rows = 0;   // kChunkRows
     t.persistent = 1;   // persistent workgroups over the work queue
-    t.tile_group = 16;  // 16 tiles of a DM block per item group
+    t.tile_group = 64;  // 64 tiles of a DM block per item group
     return t;
 }
 
@@ -322,7 +322,7 @@ int tune_search(const mdsm_tuning &start, const std::function<int(const mdsm_tun
             best = t;
         }
     }
-    const uint32_t orders[4][2] = {{0, 1}, {1, 1}, {1, 16}, {1, 64}};
+    const uint32_t orders[4][2] = {{0, 1}, {1, 16}, {1, 64}, {1, 128}};
     const mdsm_tuning base = best;
     for (int i = 0; i < 4 && !rc; ++i) {
         mdsm_tuning t = base;
