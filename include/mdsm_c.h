@@ -171,9 +171,9 @@ int mdsm_ddplan(const mdsm_ddplan_params *params, mdsm_pass *passes, uint32_t ca
  * src/mdsm/dedispersion_thread.cu:69-79). */
 typedef struct {
     uint32_t chunk_rows;  /* channels per double-buffered LDS chunk, 2..8;
-                             0 = 6, fewer where a window is too wide */
+                             0 = 8, fewer where a window is too wide */
     uint32_t lds_budget;  /* LDS bytes per workgroup for staged channels;
-                             0 = 65536 (the most allowed) */
+                             0 = 73728 (two workgroups per CU); at most 162816 */
     uint32_t persistent;  /* 1: persistent workgroups over a device work queue */
     uint32_t grid;        /* persistent workgroups; 0 = 8 per compute unit */
     uint32_t tile_group;  /* work order: consecutive items cover this many time

@@ -606,6 +606,14 @@ hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s)
     a.group = std::max(1, L.group);
     for (int i = 0; i < L.njobs; ++i) a.jobs[i] = L.jobs[i];
     const size_t lds = (size_t)L.lds_bytes;
+    if (lds > (size_t)kLdsMax) return hipErrorInvalidValue;
+    static bool lds_attr = false;  // allow the full gfx950 LDS to one workgroup, once
+    if (!lds_attr) {
+        hipError_t e = hipFuncSetAttribute((const void *)dedisperse_kernel,
+                                           hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax);
+        if (e != hipSuccess) return e;
+        lds_attr = true;
+    }
     unsigned grid = (unsigned)L.items;
     if (L.counter) {
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, L.counter, 1u, 0u);

@@ -88,8 +88,9 @@ inline int32_t lds_offset(int dtype, int W, int32_t o)
     return (o & 3) * K * 16 + (o >> 2) * 16;
 }
 inline bool int_path(int dtype) { return dtype == DT_U8 || dtype == DT_U16; }
-constexpr int kLdsBudget = 65536;  // bytes of LDS per workgroup
-constexpr int kChunkRows = 6;      // default channels per LDS chunk (2..8)
+constexpr int kLdsBudget = 73728;  // default LDS bytes per workgroup (two per CU)
+constexpr int kLdsMax = 163840 - 1024;  // gfx950: 160 KiB per CU, less the kernel's static LDS
+constexpr int kChunkRows = 8;      // default channels per LDS chunk (2..8)
 
 // A run of blocks tiled over the same number of output tiles; flat work
 // items of a segment are ordered tile-major (all blocks of tile 0 first).

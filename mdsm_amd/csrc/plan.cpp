@@ -183,7 +183,7 @@ int add_decimation(mdsm_plan *p, int k, uint64_t &work)
 }
 
 // Channels per LDS chunk of every block and each launch's dynamic LDS, from
-// the plan's tuning: the requested rows (2..8, 6 by default), or fewer where
+// the plan's tuning: the requested rows (2..8, 8 by default), or fewer where
 // a block's window is too wide for the LDS budget.  Chunks are
 // double-buffered.
 void apply_tuning(mdsm_plan *p)
@@ -805,8 +805,8 @@ extern "C" int mdsm_plan_set_tuning(mdsm_plan *p, const mdsm_tuning *t)
     if (!p || !t) return fail(MDSM_ERR_INVALID, "NULL argument");
     if (t->chunk_rows == 1 || t->chunk_rows > 8)
         return fail(MDSM_ERR_INVALID, "chunk_rows must be 0 or 2..8");
-    if (t->lds_budget > (uint32_t)kLdsBudget)
-        return fail(MDSM_ERR_INVALID, "lds_budget must be <= 65536");
+    if (t->lds_budget > (uint32_t)kLdsMax)
+        return fail(MDSM_ERR_INVALID, "lds_budget must be <= 162816");
     if (t->lds_budget && t->lds_budget < 4096)
         return fail(MDSM_ERR_INVALID, "lds_budget must be 0 or >= 4096");
     if (t->persistent > 1) return fail(MDSM_ERR_INVALID, "persistent must be 0 or 1");
