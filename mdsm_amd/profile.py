@@ -31,13 +31,17 @@ import sys
 COPY_C, COPY_S = 1024, 1 << 18          # float32 [1024][262144] = 1 GiB
 
 
-def _steps(workload: str, steps: int) -> None:
+def _steps(workload: str, steps: int, tile_group=None) -> None:
     sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
     import mdsm_amd as M
     import bench
     w = bench.WORKLOADS[workload]
     x, _ = bench.make_input(M, w)
     plan = bench.make_plan(M, w)
+    if tile_group is not None:
+        t = plan.tuning()
+        t["tile_group"] = tile_group
+        plan.set_tuning(**t)
     din = M.DeviceBuffer.from_host(x)
     dout = M.DeviceBuffer(plan.info.out_elems * 4)
     dw = M.DeviceBuffer(max(plan.info.work_bytes, 256))
@@ -187,6 +191,8 @@ def main() -> None:
     s = sub.add_parser("steps")
     s.add_argument("--workload", default="cfg2")
     s.add_argument("--steps", type=int, default=3)
+    s.add_argument("--tile-group", type=int, default=None,
+                   help="override the plan's tile group (items sharing one tile window)")
     sub.add_parser("copy")
     z = sub.add_parser("summarize")
     z.add_argument("--dir", required=True)
@@ -194,7 +200,7 @@ def main() -> None:
     z.add_argument("--kernel", default="dedisperse_kernel")
     args = ap.parse_args()
     if args.cmd == "steps":
-        _steps(args.workload, args.steps)
+        _steps(args.workload, args.steps, args.tile_group)
     elif args.cmd == "copy":
         _copy()
     else:
