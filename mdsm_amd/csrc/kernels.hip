@@ -389,7 +389,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 const cint32_t *ri = as_const(grow + i * kDB);
 #pragma unroll
                 for (int r = 0; r < kRPW; ++r) oi[r] = ri[r];
-                const char *rowi = lane_base + i * rb;
+                const char *rowi = lane_base;  // the table entry holds the row's place
                 float4 x[kRPW];
 #pragma unroll
                 for (int r = 0; r < kRPW; ++r) {
@@ -419,8 +419,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                     oi[r] = ri[r];
                     oj[r] = ri[kDB + r];
                 }
-                const char *rowi = lane_base + i * rb;
-                const char *rowj = rowi + rb;
+                const char *rowi = lane_base, *rowj = lane_base;
                 uint2 x[kRPW], y[kRPW];
 #pragma unroll
                 for (int r = 0; r < kRPW; ++r) {
@@ -443,7 +442,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 const cint32_t *ri = as_const(grow + i * kDB);
 #pragma unroll
                 for (int r = 0; r < kRPW; ++r) oi[r] = ri[r];
-                const char *rowi = lane_base + i * rb;
+                const char *rowi = lane_base;
 #pragma unroll
                 for (int r = 0; r < kRPW; ++r) add_entry(r, *(const uint2 *)(rowi + oi[r]));
             }

@@ -178,7 +178,8 @@ struct mdsm_plan {
     std::vector<mdsm::DecimStep> decim;
     std::vector<mdsm::BlockDesc> blocks;
     std::vector<mdsm::SegDesc> segs;
-    std::vector<int32_t> rel, abase;
+    std::vector<int32_t> rel0;         // per-(row, trial) phase offsets, row place not included
+    std::vector<int32_t> rel, abase;   // rel = rel0 + (row mod chunk rows) * row bytes (device copy)
     std::vector<mdsm::Stage> stages;   // launched in order
     mdsm::KernelTables dev{};
     mdsm_tuning tune{};

@@ -137,18 +137,19 @@ int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
         b.dst = s.dst;
         b.dtype = s.dtype;
         b.flush = int_path(s.dtype) ? std::max(1, 65535 / std::max(1, s.maxval)) : 0;
-        b.tab_off = (int32_t)p->rel.size();
+        b.tab_off = (int32_t)p->rel0.size();
         b.base_off = (int32_t)p->abase.size();
-        if (p->rel.size() + (size_t)s.nrows * kDB > (size_t)std::numeric_limits<int32_t>::max())
+        if (p->rel0.size() + (size_t)s.nrows * kDB > (size_t)std::numeric_limits<int32_t>::max())
             return fail(MDSM_ERR_INVALID, "plan shift tables exceed 2^31 entries");
         // Offsets are stored as byte offsets within the block's four-phase LDS
-        // row (phase o&3, entry o>>2), so the kernel adds them to a lane
-        // address without further arithmetic.
+        // row (phase o&3, entry o>>2); apply_tuning adds the row's place in
+        // its chunk, so the kernel adds them to a lane address without
+        // further arithmetic.
         for (int i = 0; i < s.nrows; ++i) {
             p->abase.push_back(base[i]);
             for (int j = 0; j < kDB; ++j) {
                 const int32_t o = j < nt ? s.sh(j0 + j, i) - base[i] : 0;
-                p->rel.push_back(lds_offset(s.dtype, b.W, o));
+                p->rel0.push_back(lds_offset(s.dtype, b.W, o));
             }
         }
         st.wmax = std::max(st.wmax, b.W);
@@ -157,7 +158,7 @@ int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
     }
     // 8 zero rows after every block's table: the integer path stages 8 rows of
     // offsets per chunk without a bounds test
-    for (int i = 0; i < 8 * kDB; ++i) p->rel.push_back(0);
+    for (int i = 0; i < 8 * kDB; ++i) p->rel0.push_back(0);
     seg.nblocks = (int32_t)p->blocks.size() - seg.first_block;
     seg.item_start = st.items;
     st.items += (int64_t)seg.nblocks * seg.ntiles;
@@ -200,6 +201,16 @@ void apply_tuning(mdsm_plan *p)
                 b.ccr = std::max(2, std::min({8, want, budget / (2 * rb)}));
                 st.lds_bytes = std::max(st.lds_bytes, 2 * b.ccr * rb);
             }
+        }
+    }
+    // table entries of row i carry the row's place in its chunk buffer
+    p->rel = p->rel0;
+    for (const BlockDesc &b : p->blocks) {
+        const int rb = lds_row_bytes(b.dtype, b.W);
+        for (int i = 0; i < b.nrows; ++i) {
+            int32_t *row = &p->rel[(size_t)b.tab_off + (size_t)i * kDB];
+            const int32_t add = (i % b.ccr) * rb;
+            for (int j = 0; j < kDB; ++j) row[j] += add;
         }
     }
 }
@@ -815,14 +826,17 @@ extern "C" int mdsm_plan_set_tuning(mdsm_plan *p, const mdsm_tuning *t)
     p->tune = *t;
     apply_tuning(p);
     if (p->uploaded) {
-        // only chunk sizes changed: refresh the block table in place
+        // only chunk sizes changed: refresh the block and offset tables in place
         int cur = 0;
         hipError_t e = hipGetDevice(&cur);
         if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
         if (cur != p->device) return fail(MDSM_ERR_STATE, "plan belongs to another device");
         e = hipMemcpy(p->dev.blocks, p->blocks.data(), p->blocks.size() * sizeof(BlockDesc),
                       hipMemcpyHostToDevice);
-        if (e != hipSuccess) return hip_fail(e, "updating the block table");
+        if (e == hipSuccess)
+            e = hipMemcpy(p->dev.rel, p->rel.data(), p->rel.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "updating the block and offset tables");
     }
     return MDSM_OK;
 }
