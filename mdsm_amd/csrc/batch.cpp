@@ -97,10 +97,9 @@ int enqueue(mdsm_batch *b, const void *const *ins, const int64_t *strides, float
         const mdsm_plan *p = b->plans[i];
         wk[i] = (works && works[i]) ? works[i] : b->own_work[i];
         if (p->work_bytes && !wk[i]) return fail(MDSM_ERR_STATE, "plan workspace missing");
-        for (const DecimStep &d : p->decim) {
-            hipError_t e = launch_decimate(ins[i], d.dtype_in, p->obs.nchans, strides[i],
-                                           p->obs.nsamp_in, (uint32_t)d.k,
-                                           (char *)wk[i] + d.work_off, d.dtype_out, d.stride, s);
+        if (!p->decim.empty()) {
+            hipError_t e = launch_decimate_steps(ins[i], p->obs.nchans, strides[i], p->obs.nsamp_in,
+                                                 p->decim.data(), p->decim.size(), (char *)wk[i], s);
             if (e != hipSuccess) return hip_fail(e, "batch decimate launch");
         }
     }

@@ -677,6 +677,124 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// uint8 -> uint16 k-sums for every k in {2, 4, 8, 16} a plan needs, in one
+// pass: a thread loads 16 consecutive samples (one 16-byte load) and forms the
+// pairwise tree s2, s4, s8, s16.  Integer sums are exact in any order, so each
+// level equals the sequential k-sum.  Output i of level k covers input samples
+// [k i, k i + k); only i < floor(nsamp / k) is written.
+struct PyrLevel {
+    uint16_t *out;
+    int64_t stride;   // elements per output row
+    uint32_t nout;
+    uint32_t lg;      // log2 k, 1..4
+};
+struct PyrArgs {
+    const uint8_t *in;
+    int64_t in_stride;
+    uint32_t nsamp;
+    int32_t nlev;
+    int32_t vec_in;   // 16-byte aligned rows: vector loads
+    PyrLevel lev[4];
+};
+
+__global__ void __launch_bounds__(256) decimate_u8_pyramid_kernel(PyrArgs a)
+{
+    const uint32_t g = blockIdx.x * 256u + threadIdx.x;
+    const int64_t c = blockIdx.y;
+    const uint32_t col = g * 16u;
+    if (col >= a.nsamp) return;
+    const uint8_t *row = a.in + c * a.in_stride;
+    uint32_t w[4];
+    if (a.vec_in && col + 16u <= a.nsamp) {
+        const uint4 v = *(const uint4 *)(row + col);
+        w[0] = v.x;
+        w[1] = v.y;
+        w[2] = v.z;
+        w[3] = v.w;
+    } else {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            w[q] = 0u;
+#pragma unroll
+            for (int b = 0; b < 4; ++b) {
+                const uint32_t t = col + 4u * q + b;
+                if (t < a.nsamp) w[q] |= (uint32_t)row[t] << (8 * b);
+            }
+        }
+    }
+    // s2 as 16-bit pairs: p[q] = (b0+b1) | (b2+b3) << 16 of dword q
+    uint32_t s2[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s2[q] = (w[q] & 0x00ff00ffu) + ((w[q] >> 8) & 0x00ff00ffu);
+    uint32_t s4[4];  // 4-sums of samples 4q..4q+3 as plain integers
+#pragma unroll
+    for (int q = 0; q < 4; ++q) s4[q] = (s2[q] & 0xffffu) + (s2[q] >> 16);
+    const uint32_t s8[2] = {s4[0] + s4[1], s4[2] + s4[3]};
+    const uint32_t s16 = s8[0] + s8[1];
+    for (int l = 0; l < a.nlev; ++l) {
+        const PyrLevel L = a.lev[l];
+        uint16_t *o = L.out + c * L.stride;
+        const uint32_t i0 = col >> L.lg;  // first output of this thread at level l
+        const uint32_t n = 16u >> L.lg;   // outputs per thread
+        if (i0 + n <= L.nout) {
+            if (L.lg == 1)
+                *(uint4 *)(o + i0) = make_uint4(s2[0], s2[1], s2[2], s2[3]);
+            else if (L.lg == 2)
+                *(uint2 *)(o + i0) = make_uint2(s4[0] | (s4[1] << 16), s4[2] | (s4[3] << 16));
+            else if (L.lg == 3)
+                *(uint32_t *)(o + i0) = s8[0] | (s8[1] << 16);
+            else
+                o[i0] = (uint16_t)s16;
+        } else {
+            for (uint32_t j = 0; j < n && i0 + j < L.nout; ++j) {
+                uint32_t v;
+                if (L.lg == 1)
+                    v = (j & 1) ? (s2[j >> 1] >> 16) : (s2[j >> 1] & 0xffffu);
+                else if (L.lg == 2)
+                    v = s4[j];
+                else if (L.lg == 3)
+                    v = s8[j];
+                else
+                    v = s16;
+                o[i0 + j] = (uint16_t)v;
+            }
+        }
+    }
+}
+
+hipError_t launch_decimate_steps(const void *in, uint32_t nchans, int64_t in_stride,
+                                 uint32_t nsamp, const DecimStep *steps, size_t nsteps, char *work,
+                                 hipStream_t s)
+{
+    PyrArgs a{};
+    a.in = (const uint8_t *)in;
+    a.in_stride = in_stride;
+    a.nsamp = nsamp;
+    a.vec_in = ((uintptr_t)in % 16 == 0 && in_stride % 16 == 0) ? 1 : 0;
+    for (size_t i = 0; i < nsteps; ++i) {
+        const DecimStep &d = steps[i];
+        const int k = d.k;
+        const bool pyr = d.dtype_in == DT_U8 && d.dtype_out == DT_U16 &&
+                         (k == 2 || k == 4 || k == 8 || k == 16) && a.nlev < 4 &&
+                         ((uintptr_t)(work + d.work_off) % 16) == 0 && d.stride % 8 == 0;
+        if (pyr) {
+            PyrLevel &L = a.lev[a.nlev++];
+            L.out = (uint16_t *)(work + d.work_off);
+            L.stride = d.stride;
+            L.nout = nsamp / (uint32_t)k;
+            L.lg = k == 2 ? 1 : k == 4 ? 2 : k == 8 ? 3 : 4;
+        } else {
+            hipError_t e = launch_decimate(in, d.dtype_in, nchans, in_stride, nsamp, (uint32_t)k,
+                                           work + d.work_off, d.dtype_out, d.stride, s);
+            if (e != hipSuccess) return e;
+        }
+    }
+    if (a.nlev == 0 || nchans == 0 || nsamp == 0) return hipSuccess;
+    dim3 grid(((nsamp + 15) / 16 + 255) / 256, nchans);
+    hipLaunchKernelGGL(decimate_u8_pyramid_kernel, grid, dim3(256), 0, s, a);
+    return hipGetLastError();
+}
+
 hipError_t launch_decimate(const void *in, int dtype_in, uint32_t nchans, int64_t in_stride,
                            uint32_t nsamp, uint32_t k, void *out, int dtype_out,
                            int64_t out_stride, hipStream_t s)

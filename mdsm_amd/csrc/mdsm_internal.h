@@ -227,6 +227,11 @@ hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s);
 hipError_t launch_decimate(const void *in, int dtype_in, uint32_t nchans, int64_t in_stride,
                            uint32_t nsamp, uint32_t k, void *out, int dtype_out,
                            int64_t out_stride, hipStream_t s);
+// Every decimation of a plan (steps[0..n)), reading `in` once for the
+// uint8 -> uint16 steps with k in {2, 4, 8, 16}; outputs at work + work_off.
+hipError_t launch_decimate_steps(const void *in, uint32_t nchans, int64_t in_stride,
+                                 uint32_t nsamp, const DecimStep *steps, size_t nsteps, char *work,
+                                 hipStream_t s);
 hipError_t launch_clip_u8(const uint8_t *in, int64_t in_stride, uint32_t nchans, int64_t t0,
                           uint32_t n, float nsigma, uint8_t *out, int64_t out_stride,
                           int32_t *med, int32_t *mad, int32_t *ncl, hipStream_t s);

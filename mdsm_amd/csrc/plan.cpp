@@ -797,10 +797,10 @@ extern "C" int mdsm_dedisperse(mdsm_plan *p, const void *d_in, int64_t in_stride
         }
         work = p->own_work;
     }
-    for (const DecimStep &d : p->decim) {
+    if (!p->decim.empty()) {
         TimedLaunch tl("decimate", s);
-        e = launch_decimate(d_in, d.dtype_in, p->obs.nchans, in_stride, p->obs.nsamp_in,
-                            (uint32_t)d.k, (char *)work + d.work_off, d.dtype_out, d.stride, s);
+        e = launch_decimate_steps(d_in, p->obs.nchans, in_stride, p->obs.nsamp_in, p->decim.data(),
+                                  p->decim.size(), (char *)work, s);
         if (e != hipSuccess) return hip_fail(e, "decimate launch");
     }
     for (const Stage &st : p->stages) {

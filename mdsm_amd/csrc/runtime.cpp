@@ -276,10 +276,10 @@ int launch_batch(mdsm_server *s, Batch *bt)
                               (int64_t)(N * es), (int64_t)(N * es), (uint32_t)C, st);
             if (e != hipSuccess) return hip_fail(e, "server block placement");
         }
-        for (const DecimStep &d : b->plan->decim) {
+        if (!b->plan->decim.empty()) {
             TimedLaunch tl("decimate", st);
-            e = launch_decimate(b->ring, d.dtype_in, (uint32_t)C, (int64_t)S, (uint32_t)S,
-                                (uint32_t)d.k, j->work + d.work_off, d.dtype_out, d.stride, st);
+            e = launch_decimate_steps(b->ring, (uint32_t)C, (int64_t)S, (uint32_t)S,
+                                      b->plan->decim.data(), b->plan->decim.size(), j->work, st);
             if (e != hipSuccess) return hip_fail(e, "server decimate");
         }
     }

@@ -92,6 +92,22 @@ def test_multi_range_decimation_u8(gpu):
         _check_range(plan, r, x, outs[r], "u8")
 
 
+@pytest.mark.parametrize("pad", [0, 5])
+def test_decimation_pyramid_u8(gpu, pad):
+    """uint8 ranges at k = 2, 4, 8, 16 (one fused decimation pass) and k = 3
+    (its own pass), a block length that is no multiple of 16, and rows that
+    are (pad 0: stride 10016) or are not (pad 5) 16-byte aligned."""
+    M = gpu
+    C, S = 40, 10011
+    print(f"case decimation_pyramid_u8 pad={pad}", flush=True)
+    x = M.synth_filterbank(C, 10016 + pad, "u8", seed=23, mean=128, sigma=40)
+    plan = _plan(M, C, S, "u8", [(0.0, 2.0, 20, 1), (40.0, 3.0, 12, 2), (80.0, 4.0, 10, 4),
+                                 (120.0, 6.0, 9, 8), (180.0, 8.0, 7, 16), (240.0, 9.0, 5, 3)])
+    outs = plan.dedisperse_host(x)
+    for r in range(6):
+        _check_range(plan, r, x[:, :S], outs[r], "u8")
+
+
 @pytest.mark.parametrize("dtype", ["f16", "f32"])
 def test_multi_range_decimation_float(gpu, dtype):
     M = gpu
