@@ -185,6 +185,20 @@ def read_traffic(path: str) -> dict | None:
         return None
 
 
+def kernel_table(args) -> None:
+    """Per-kernel statistics of one --kernel-trace --stats run (no counters)."""
+    rows = kernel_stats(_db(args.dir))
+    print(f"# rocprofv3 kernel statistics: {args.title}\n")
+    print("Command (repository root, one GPU):\n")
+    print("```")
+    print(args.command)
+    print("```\n")
+    print("| kernel | calls | total ms | average ms | % |")
+    print("|---|---|---|---|---|")
+    for name, calls, total, avg, pct in rows:
+        print(f"| `{name[:70]}` | {calls} | {total / 1e3:.3f} | {avg / 1e3:.4f} | {pct:.1f} |")  # us -> ms
+
+
 def main() -> None:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     sub = ap.add_subparsers(dest="cmd", required=True)
@@ -198,11 +212,17 @@ def main() -> None:
     z.add_argument("--dir", required=True)
     z.add_argument("--workload", default="cfg2")
     z.add_argument("--kernel", default="dedisperse_kernel")
+    kt = sub.add_parser("kernels")
+    kt.add_argument("--dir", required=True)
+    kt.add_argument("--title", default="")
+    kt.add_argument("--command", default="")
     args = ap.parse_args()
     if args.cmd == "steps":
         _steps(args.workload, args.steps, args.tile_group)
     elif args.cmd == "copy":
         _copy()
+    elif args.cmd == "kernels":
+        kernel_table(args)
     else:
         summarize(args)
 
