@@ -160,6 +160,66 @@ def test_config1_full_size(gpu):
     assert abs(d - 284) <= 3 and abs(t - 30000) <= 4
 
 
+def _check_integer_plan_full(M, plan, x, nchk=4096):
+    """uint8 plan at full size: (a) every DM row's total over its N_r output
+    samples equals, exactly, sum_c P_c[N_r + s_dc] - P_c[s_dc] with P_c the
+    prefix sums of channel c of the (k-summed) input -- a checksum of every
+    row that moves if any shift, window or row is wrong; (b) the first nchk
+    samples of the first, middle and last row of each range are bit-exact
+    against the oracle run on the window they depend on."""
+    C, S = x.shape[0], plan.obs.nsamp_in
+    din = M.DeviceBuffer.from_host(x)
+    dout = M.DeviceBuffer(plan.info.out_elems * 4)
+    dw = M.DeviceBuffer(max(plan.info.work_bytes, 256))
+    plan.dedisperse(din, x.shape[1], dout, dw)
+    M.synchronize()
+    for r, R in enumerate(plan.ranges):
+        k, n = R.downsample, S // R.downsample
+        sh = plan.shift_table(r)
+        tot = np.zeros(R.ndms, np.int64)
+        for c in range(C):
+            xc = x[c, :n * k].reshape(n, k).sum(axis=1, dtype=np.int64)
+            P = np.concatenate(([0], np.cumsum(xc)))
+            tot += P[R.nout + sh[:, c]] - P[sh[:, c]]
+        got = dout.download(np.float32, (R.ndms, R.nout), offset=R.out_offset * 4)
+        gsum = np.add.reduce(got, axis=1, dtype=np.float64) * k
+        bad = np.flatnonzero(gsum != tot.astype(np.float64))
+        print(f"range {r}: k {k} ndms {R.ndms} nout {R.nout} maxshift {R.maxshift} "
+              f"row totals differing {bad.size}", flush=True)
+        assert bad.size == 0, f"range {r}: rows {bad[:8]} differ"
+        rows = [0, R.ndms // 2, R.ndms - 1]
+        win = np.ascontiguousarray(x[:, :(R.maxshift + nchk) * k])
+        ref = oracle.range_f32(win, k, sh[rows], nchk)
+        assert np.array_equal(got[rows, :nchk], ref), f"range {r}: sampled rows not bit-exact"
+        del got
+
+
+def test_config2_full_size(gpu):
+    """Config 2 at full size: 1024 ch x 2^20 uint8 with injected pulses,
+    ranges 0-200/0.2 (k 1), 200-500/0.5 (k 2), 500-1000/1.0 (k 4)."""
+    import bench
+    M = gpu
+    print("case config2 full size", flush=True)
+    w = bench.WORKLOADS["cfg2"]
+    x, _ = bench.make_input(M, w)
+    _check_integer_plan_full(M, bench.make_plan(M, w), x)
+
+
+def test_config5_block_full_size(gpu):
+    """One config-5 block at full size: 1024 ch x (65536 + 16176) uint8,
+    2048 DMs over 0-1000."""
+    import bench
+    M = gpu
+    print("case config5 block full size", flush=True)
+    w = bench.WORKLOADS["cfg5"]
+    S = w["nsamp"] + 16176
+    x = M.synth_filterbank(w["nchans"], S, "u8", seed=55, mean=128, sigma=16,
+                           pulses=[(400.0, 30000, 8, 3.0)])
+    plan = M.Plan.brute(w["nchans"], w["fch1"], w["foff"], w["tsamp"], S, "u8", w["ranges"])
+    assert plan.ranges[0].maxshift == 16176 and plan.ranges[0].nout == w["nsamp"]
+    _check_integer_plan_full(M, plan, x)
+
+
 def test_plain_kernel_equals_staged(gpu):
     """The reference-shaped plain float32 kernel sums in the same channel
     order as the staged path: identical outputs."""
