@@ -173,7 +173,7 @@ typedef struct {
     uint32_t chunk_rows;  /* channels per double-buffered LDS chunk, 2..8;
                              0 = 8, fewer where a window is too wide */
     uint32_t lds_budget;  /* LDS bytes per workgroup for staged channels;
-                             0 = 73728 (two workgroups per CU); at most 162816 */
+                             0 = 79872 (two workgroups per CU); at most 162816 */
     uint32_t persistent;  /* 1: persistent workgroups over a device work queue */
     uint32_t grid;        /* persistent workgroups; 0 = 8 per compute unit */
     uint32_t tile_group;  /* work order: consecutive items cover this many time

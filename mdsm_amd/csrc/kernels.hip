@@ -334,7 +334,7 @@ __device__ __forceinline__ float half_hi(uint32_t v)
 // scalar loads and issues all of that channel (pair)'s LDS reads before the
 // adds.  `one` is 1.0f, passed at run time so that x*one+acc stays a fused
 // convert-and-add.
-template <int DT>
+template <int DT, int R>
 __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
                                  int lane, char *lds, float one)
 {
@@ -350,10 +350,10 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
     const size_t bufsz = (size_t)ccr * rb;      // bytes per chunk buffer
     const int nch = (bd.nrows + ccr - 1) / ccr;
 
-    float f[kRPW][4];
-    uint32_t ae[kRPW], ao[kRPW];
+    float f[R][4];
+    uint32_t ae[R], ao[R];
 #pragma unroll
-    for (int r = 0; r < kRPW; ++r) {
+    for (int r = 0; r < R; ++r) {
         ae[r] = ao[r] = 0u;
 #pragma unroll
         for (int j = 0; j < 4; ++j) f[r][j] = 0.0f;
@@ -361,7 +361,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
     auto flush = [&]() {
         if constexpr (kInt) {
 #pragma unroll
-            for (int r = 0; r < kRPW; ++r) {
+            for (int r = 0; r < R; ++r) {
                 f[r][0] += (float)(ae[r] & 0xffffu);
                 f[r][2] += (float)(ae[r] >> 16);
                 f[r][1] += (float)(ao[r] & 0xffffu);
@@ -385,14 +385,14 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
     auto sum_chunk = [&](const char *lane_base, const int32_t *grow, int cc) {
         if constexpr (DT == DT_F32) {
             for (int i = 0; i < cc; ++i) {
-                int oi[kRPW];
+                int oi[R];
                 const cint32_t *ri = as_const(grow + i * kDB);
 #pragma unroll
-                for (int r = 0; r < kRPW; ++r) oi[r] = ri[r];
+                for (int r = 0; r < R; ++r) oi[r] = ri[r];
                 const char *rowi = lane_base;  // the table entry holds the row's place
-                float4 x[kRPW];
+                float4 x[R];
 #pragma unroll
-                for (int r = 0; r < kRPW; ++r) {
+                for (int r = 0; r < R; ++r) {
                     if constexpr (kF32Two) {
                         const float2 *q = (const float2 *)(rowi + oi[r]);
                         const float2 lo = q[0], hi = q[64];
@@ -402,7 +402,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                     }
                 }
 #pragma unroll
-                for (int r = 0; r < kRPW; ++r) {
+                for (int r = 0; r < R; ++r) {
                     f[r][0] += x[r].x;
                     f[r][1] += x[r].y;
                     f[r][2] += x[r].z;
@@ -412,22 +412,22 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         } else {
             int i = 0;
             for (; i + 1 < cc; i += 2) {
-                int oi[kRPW], oj[kRPW];
+                int oi[R], oj[R];
                 const cint32_t *ri = as_const(grow + i * kDB);
 #pragma unroll
-                for (int r = 0; r < kRPW; ++r) {
+                for (int r = 0; r < R; ++r) {
                     oi[r] = ri[r];
                     oj[r] = ri[kDB + r];
                 }
                 const char *rowi = lane_base, *rowj = lane_base;
-                uint2 x[kRPW], y[kRPW];
+                uint2 x[R], y[R];
 #pragma unroll
-                for (int r = 0; r < kRPW; ++r) {
+                for (int r = 0; r < R; ++r) {
                     x[r] = *(const uint2 *)(rowi + oi[r]);
                     y[r] = *(const uint2 *)(rowj + oj[r]);
                 }
 #pragma unroll
-                for (int r = 0; r < kRPW; ++r) {
+                for (int r = 0; r < R; ++r) {
                     if constexpr (kInt) {
                         ae[r] = ae[r] + x[r].x + y[r].x;
                         ao[r] = ao[r] + x[r].y + y[r].y;
@@ -438,13 +438,13 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 }
             }
             if (i < cc) {
-                int oi[kRPW];
+                int oi[R];
                 const cint32_t *ri = as_const(grow + i * kDB);
 #pragma unroll
-                for (int r = 0; r < kRPW; ++r) oi[r] = ri[r];
+                for (int r = 0; r < R; ++r) oi[r] = ri[r];
                 const char *rowi = lane_base;
 #pragma unroll
-                for (int r = 0; r < kRPW; ++r) add_entry(r, *(const uint2 *)(rowi + oi[r]));
+                for (int r = 0; r < R; ++r) add_entry(r, *(const uint2 *)(rowi + oi[r]));
             }
         }
     };
@@ -483,8 +483,8 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
     float *ob = (bd.dst == BUF_OUT ? a.out : (float *)a.work) + bd.out_off;
     const float kf = (float)bd.k;
 #pragma unroll
-    for (int r = 0; r < kRPW; ++r) {
-        const int j = wave * kRPW + r;
+    for (int r = 0; r < R; ++r) {
+        const int j = wave * R + r;
         if (j >= bd.ntrials) continue;
         float v[4];
 #pragma unroll
@@ -546,11 +546,20 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
     ItemCtx ctx{uniform_ptr(sg.rel), uniform_ptr(sg.abase), uniform_ptr(jb.in),
                 uniform_ptr(jb.out), uniform_ptr(jb.work), jb.in_stride};
     const int t0 = tile_i * kTile;
-    switch (bd.dtype) {
-    case DT_U8: dedisperse_lds_t<DT_U8>(ctx, bd, t0, wave, lane, lds, a.one); break;
-    case DT_U16: dedisperse_lds_t<DT_U16>(ctx, bd, t0, wave, lane, lds, a.one); break;
-    case DT_F16: dedisperse_lds_t<DT_F16>(ctx, bd, t0, wave, lane, lds, a.one); break;
-    default: dedisperse_lds_t<DT_F32>(ctx, bd, t0, wave, lane, lds, a.one); break;
+    // float blocks of at most kDB/2 trials run kRPW/2 trials per wave
+    if (bd.rpw == kRPW) {
+        switch (bd.dtype) {
+        case DT_U8: dedisperse_lds_t<DT_U8, kRPW>(ctx, bd, t0, wave, lane, lds, a.one); break;
+        case DT_U16: dedisperse_lds_t<DT_U16, kRPW>(ctx, bd, t0, wave, lane, lds, a.one); break;
+        case DT_F16: dedisperse_lds_t<DT_F16, kRPW>(ctx, bd, t0, wave, lane, lds, a.one); break;
+        default: dedisperse_lds_t<DT_F32, kRPW>(ctx, bd, t0, wave, lane, lds, a.one); break;
+        }
+    } else {  // float data only (integer blocks always run kRPW trials per wave)
+        constexpr int H = kRPW / 2;
+        if (bd.dtype == DT_F16)
+            dedisperse_lds_t<DT_F16, H>(ctx, bd, t0, wave, lane, lds, a.one);
+        else
+            dedisperse_lds_t<DT_F32, H>(ctx, bd, t0, wave, lane, lds, a.one);
     }
 }
 

@@ -66,7 +66,7 @@ struct BlockDesc {
     int32_t base_off;         // int offset of abase[nrows] in the base table
     int32_t ccr;              // rows staged per LDS chunk
     int32_t flush;            // integer path: rows summed in 16-bit lanes between flushes
-    int32_t pad;
+    int32_t rpw;              // trials per wave: kRPW, or kRPW/2 for float blocks of <= kDB/2 trials
 };
 
 // LDS bytes of one staged row: integer data (uint8/uint16) is held as four
@@ -88,7 +88,7 @@ inline int32_t lds_offset(int dtype, int W, int32_t o)
     return (o & 3) * K * 16 + (o >> 2) * 16;
 }
 inline bool int_path(int dtype) { return dtype == DT_U8 || dtype == DT_U16; }
-constexpr int kLdsBudget = 73728;  // default LDS bytes per workgroup (two per CU)
+constexpr int kLdsBudget = 79872;  // default LDS bytes per workgroup (two per CU: 2 x 78 KiB + static)
 constexpr int kLdsMax = 163840 - 1024;  // gfx950: 160 KiB per CU, less the kernel's static LDS
 constexpr int kChunkRows = 8;      // default channels per LDS chunk (2..8)
 

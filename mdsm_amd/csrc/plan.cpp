@@ -145,10 +145,17 @@ int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
         // row (phase o&3, entry o>>2); apply_tuning adds the row's place in
         // its chunk, so the kernel adds them to a lane address without
         // further arithmetic.
+        // Trial t of a block sits in the table column its wave reads: column
+        // t with kRPW trials per wave; with kRPW/2 (float blocks of <= kDB/2
+        // trials, which then keep every wave busy) column (t / R) * kRPW + t % R.
+        b.rpw = (nt <= kDB / 2 && !int_path(s.dtype)) ? kRPW / 2 : kRPW;
+        std::vector<int> col_trial(kDB, -1);
+        for (int t = 0; t < nt; ++t) col_trial[(t / b.rpw) * kRPW + t % b.rpw] = t;
         for (int i = 0; i < s.nrows; ++i) {
             p->abase.push_back(base[i]);
             for (int j = 0; j < kDB; ++j) {
-                const int32_t o = j < nt ? s.sh(j0 + j, i) - base[i] : 0;
+                const int t = col_trial[j];
+                const int32_t o = t >= 0 ? s.sh(j0 + t, i) - base[i] : 0;
                 p->rel0.push_back(lds_offset(s.dtype, b.W, o));
             }
         }
