@@ -50,6 +50,7 @@ def lib():
         L.orc_half_to_float.restype = f32
         L.orc_half_to_float.argtypes = [C.c_uint16]
         L.orc_dedisperse_ref_tm.argtypes = [vp, i32, i32, i32, vp, vp]
+        L.orc_dedisperse_ref_tm_mt.argtypes = [vp, i32, i32, i32, vp, vp, i32]
         L.orc_dedisperse_f32.argtypes = [vp, i32, i32, i64, i32, i32, vp, vp, i64]
         L.orc_dedisperse_f64.argtypes = [vp, i32, i32, i64, i32, i32, vp, vp, i64]
         L.orc_decimate_f64.argtypes = [vp, i32, i32, i64, i32, i32, vp]
@@ -107,6 +108,16 @@ def dedisperse_ref_tm(x_tm, shift, nsamp):
     ndms, nchans = shift.shape
     out = np.empty((nsamp, ndms), np.float32)
     lib().orc_dedisperse_ref_tm(_p(x_tm), nchans, nsamp, ndms, _p(shift), _p(out))
+    return out
+
+
+def dedisperse_ref_tm_mt(x_tm, shift, nsamp, nthreads):
+    """dedisperse_ref_tm on nthreads OpenMP threads (same float sums)."""
+    x_tm = np.ascontiguousarray(x_tm, np.float32)
+    shift = np.ascontiguousarray(shift, np.int32)
+    ndms, nchans = shift.shape
+    out = np.empty((nsamp, ndms), np.float32)
+    lib().orc_dedisperse_ref_tm_mt(_p(x_tm), nchans, nsamp, ndms, _p(shift), _p(out), nthreads)
     return out
 
 

@@ -152,6 +152,28 @@ void orc_dedisperse_ref_tm(const float *in_tm, int nchans, int nsamp,
     }
 }
 
+/* The same loop over (trial, 1024-sample run) pairs shared among nthreads
+ * OpenMP threads; every output is the same channel-order float sum, so the
+ * result equals orc_dedisperse_ref_tm bit for bit. */
+void orc_dedisperse_ref_tm_mt(const float *in_tm, int nchans, int nsamp,
+                              int ndms, const int32_t *shift, float *out_tm, int nthreads)
+{
+    const int runs = (nsamp + 1023) / 1024;
+    const long jobs = (long)ndms * runs;
+#pragma omp parallel for schedule(dynamic, 4) num_threads(nthreads)
+    for (long q = 0; q < jobs; q++) {
+        const int n = (int)(q / runs), s0 = (int)(q % runs) * 1024;
+        const int s1 = s0 + 1024 < nsamp ? s0 + 1024 : nsamp;
+        const int32_t *sh = shift + (size_t)n * nchans;
+        for (int s = s0; s < s1; s++) {
+            float value = 0.0f;
+            for (int c = 0; c < nchans; c++)
+                value += in_tm[((size_t)s + sh[c]) * nchans + c];
+            out_tm[(size_t)s * ndms + n] = value;
+        }
+    }
+}
+
 /* Same sum with the channel-major input [c][S] and DM-major output [d][t]
  * that the GPU path uses (src/mdsm/dedispersion_kernel.cu:51-77 layout),
  * float accumulation in channel order as cpu_dedisp.c:92-93. */

@@ -73,6 +73,8 @@ double orc_load(const void *in, int dtype, size_t idx);
 /* --- brute force ----------------------------------------------------- */
 void orc_dedisperse_ref_tm(const float *in_tm, int nchans, int nsamp,
                            int ndms, const int32_t *shift, float *out_tm);
+void orc_dedisperse_ref_tm_mt(const float *in_tm, int nchans, int nsamp,
+                              int ndms, const int32_t *shift, float *out_tm, int nthreads);
 void orc_dedisperse_f32(const void *in, int dtype, int nchans,
                         int64_t in_stride, int nout, int ndms,
                         const int32_t *shift, float *out, int64_t out_stride);

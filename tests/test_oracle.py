@@ -82,6 +82,10 @@ def test_dedisperse_matches_reference_loop_golden():
     # channel-major restatement, same float order
     out_cm = oracle.dedisperse_f32(np.ascontiguousarray(g["x_tm"].T), g["shift"], N)
     assert np.array_equal(out_cm, g["out_tm"].T)
+    # the multi-threaded loop (the bench's all-cores CPU figure), same sums
+    for nthreads in (1, 3, 8):
+        assert np.array_equal(oracle.dedisperse_ref_tm_mt(g["x_tm"], g["shift"], N, nthreads),
+                              g["out_tm"])
 
 
 @needs_ref
