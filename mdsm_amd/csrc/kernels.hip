@@ -176,15 +176,35 @@ __device__ __forceinline__ uint32_t perm(uint32_t hi, uint32_t lo, uint32_t sel)
     return __builtin_amdgcn_perm(hi, lo, sel);
 }
 
+// Global-address-space views.  Loads through generic pointers compile to
+// FLAT instructions, which count against LGKM_CNT as well as VM_CNT: every
+// s_waitcnt lgkmcnt(0) for a scalar (offset) load would then also wait for
+// the next chunk's prefetch.  Through these views they are global_load_*.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define MDSM_GLOBAL_AS __attribute__((address_space(1)))
+#else
+#define MDSM_GLOBAL_AS
+#endif
+template <typename T>
+__device__ __forceinline__ const MDSM_GLOBAL_AS T *gptr(const void *p)
+{
+    return (const MDSM_GLOBAL_AS T *)(p);
+}
+template <typename T>
+__device__ __forceinline__ MDSM_GLOBAL_AS T *gptr_w(void *p)
+{
+    return (MDSM_GLOBAL_AS T *)(p);
+}
+
 template <int DT>
 __device__ __forceinline__ uint32_t load_u(const void *base, int64_t i)
 {
     if constexpr (DT == DT_U8)
-        return ((const uint8_t *)base)[i];
+        return gptr<uint8_t>(base)[i];
     else if constexpr (DT == DT_F32)
-        return ((const uint32_t *)base)[i];
+        return gptr<uint32_t>(base)[i];
     else
-        return ((const uint16_t *)base)[i];
+        return gptr<uint16_t>(base)[i];
 }
 
 #ifndef MDSM_F32_2PH
@@ -211,7 +231,7 @@ __device__ __forceinline__ void fetch8(const void *src, int64_t rowbase, int col
     constexpr int n = Layout<DT>::raw;
     constexpr int es = Layout<DT>::es;
     if (aligned && col + 8 <= in_len) {
-        const uint32_t *p = (const uint32_t *)((const char *)src + (rowbase + col) * es);
+        const MDSM_GLOBAL_AS uint32_t *p = gptr<uint32_t>((const char *)src + (rowbase + col) * es);
 #pragma unroll
         for (int i = 0; i < n; ++i) raw[i] = p[i];
     } else {
@@ -290,7 +310,7 @@ __device__ __forceinline__ void fetch_chunk(PreRegs<DT> &pre, const void *src, i
         const int i = wave + kWaves * rr;
         if (i >= cc) break;
         const int64_t rowbase = in_off + (row0 + i) * rs;
-        const int col0 = ab[i] + t0;
+        const int col0 = ab[i] + t0;  // a vector load: a scalar one would tie the prefetch to LGKM waits
         const bool aligned = ((rowbase * es) & 3) == 0;
 #pragma unroll
         for (int it = 0; it < kPfIters; ++it) {
@@ -489,15 +509,16 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         float v[4];
 #pragma unroll
         for (int q = 0; q < 4; ++q) v[q] = bd.k > 1 ? __fdiv_rn(f[r][q], kf) : f[r][q];
-        float *o = ob + (int64_t)j * bd.out_trial_stride;
+        float *og = ob + (int64_t)j * bd.out_trial_stride;
+        MDSM_GLOBAL_AS float *o = gptr_w<float>(og);
         if constexpr (DT == DT_F32 && kF32Two) {
             // samples t0 + 2L + {0, 1} and t0 + 128 + 2L + {0, 1}
-            const bool al = (((uintptr_t)(o + t0)) & 7) == 0;
+            const bool al = (((uintptr_t)(og + t0)) & 7) == 0;
 #pragma unroll
             for (int h = 0; h < 2; ++h) {
                 const int t = t0 + 128 * h + 2 * lane;
                 if (al && t + 1 < bd.nout) {
-                    *(float2 *)(o + t) = make_float2(v[2 * h], v[2 * h + 1]);
+                    *(MDSM_GLOBAL_AS float2 *)(o + t) = make_float2(v[2 * h], v[2 * h + 1]);
                 } else {
                     if (t < bd.nout) o[t] = v[2 * h];
                     if (t + 1 < bd.nout) o[t + 1] = v[2 * h + 1];
@@ -505,9 +526,9 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
             }
         } else {
             const int t = t0 + 4 * lane;
-            const bool al = (((uintptr_t)(o + t0)) & 15) == 0;
+            const bool al = (((uintptr_t)(og + t0)) & 15) == 0;
             if (al && t + 3 < bd.nout) {
-                *(float4 *)(o + t) = make_float4(v[0], v[1], v[2], v[3]);
+                *(MDSM_GLOBAL_AS float4 *)(o + t) = make_float4(v[0], v[1], v[2], v[3]);
             } else {
 #pragma unroll
                 for (int q = 0; q < 4; ++q)
