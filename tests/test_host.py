@@ -118,6 +118,33 @@ def test_subband_validation():
                        [dict(dmstep=1.0, sub_dmstep=4.0, ncalls=2, calldms=4)])
 
 
+@pytest.mark.parametrize("bad", [dict(chunk_rows=1), dict(chunk_rows=9), dict(lds_budget=200000),
+                                 dict(lds_budget=100), dict(persistent=2), dict(grid=1 << 21),
+                                 dict(tile_group=5000)])
+def test_tuning_validation(bad):
+    p = M.Plan.brute(64, 1500.0, -1.0, 64e-6, 4096, "u8", [(0.0, 1.0, 10, 1)])
+    before = p.tuning()
+    t = dict(before)
+    t.update(bad)
+    with pytest.raises(M.MdsmError) as ei:
+        p.set_tuning(**t)
+    assert ei.value.code == -1
+    assert p.tuning() == before
+
+
+def test_tuning_round_trip_keeps_plan_info():
+    p = M.Plan.brute(256, 1500.0, -0.390625, 64e-6, 20000, "u8",
+                     [(0.0, 0.5, 100, 1), (50.0, 1.0, 40, 2)])
+    info = (p.info.out_elems, p.info.ndms_total, p.info.maxshift, p.info.nblocks)
+    for t in (dict(chunk_rows=2), dict(chunk_rows=6, lds_budget=40000, persistent=0),
+              dict(chunk_rows=8, lds_budget=162816, tile_group=128, grid=1000)):
+        full = dict(chunk_rows=0, lds_budget=0, persistent=1, grid=0, tile_group=64)
+        full.update(t)
+        p.set_tuning(**full)
+        assert p.tuning() == full
+        assert (p.info.out_elems, p.info.ndms_total, p.info.maxshift, p.info.nblocks) == info
+
+
 def test_synth_is_seeded_and_thread_independent():
     a = M.synth_filterbank(37, 1001, "f32", seed=9, nthreads=1)
     b = M.synth_filterbank(37, 1001, "f32", seed=9, nthreads=5)
