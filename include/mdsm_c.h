@@ -175,7 +175,8 @@ typedef struct {
     uint32_t lds_budget;  /* LDS bytes per workgroup for staged channels;
                              0 = 79872 (two workgroups per CU); at most 162816 */
     uint32_t persistent;  /* 1: persistent workgroups over a device work queue */
-    uint32_t grid;        /* persistent workgroups; 0 = 8 per compute unit */
+    uint32_t grid;        /* persistent workgroups; 0 = 8 per compute unit (a launch whose
+                             items fit in the grid runs one workgroup per item) */
     uint32_t tile_group;  /* work order: consecutive items cover this many time
                              tiles of one DM block before the next block
                              (0 or 1: every block of a tile, then the next tile) */
@@ -419,7 +420,9 @@ typedef struct {
     uint64_t hbm_budget;    /* device bytes the server may hold (0: 90% of free memory) */
     uint32_t max_inflight;  /* blocks on the device at once (0: 16) */
     uint32_t max_batch;     /* blocks per dedispersion launch (0: 16; <= 64) */
-    uint32_t persistent;    /* 1: persistent workgroups pulling from a device work counter */
+    uint32_t persistent;    /* 1: persistent workgroups (4 per CU) pulling from a device work
+                               counter; > 1: that many workgroups.  A launch whose items fit
+                               in the grid runs one workgroup per item instead */
     uint32_t queue_depth;   /* pushed-but-unadmitted blocks per beam (0: 4) */
 } mdsm_server_config;
 

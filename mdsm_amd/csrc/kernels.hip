@@ -644,9 +644,15 @@ hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s)
         lds_attr = true;
     }
     unsigned grid = (unsigned)L.items;
-    if (L.counter) {
+    // A persistent grid at least as large as the queue would give every
+    // workgroup one item anyway: launch one workgroup per item instead, which
+    // keeps the XCD-aware item order and needs no counter.
+    const int64_t pgrid = L.grid > 0 ? L.grid : 2048;
+    if (L.counter && L.items > pgrid) {
         hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, L.counter, 1u, 0u);
-        grid = (unsigned)std::min<int64_t>(L.grid > 0 ? L.grid : 2048, L.items);
+        grid = (unsigned)pgrid;
+    } else {
+        a.counter = nullptr;
     }
     hipLaunchKernelGGL(dedisperse_kernel, dim3(grid), dim3(kThreads), lds, s, a);
     return hipGetLastError();

@@ -337,7 +337,7 @@ int launch_batch(mdsm_server *s, Batch *bt)
             if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
                 hipSuccess)
                 cus = 256;
-            L.grid = cus * 4;
+            L.grid = s->cfg.persistent > 1 ? (int32_t)s->cfg.persistent : cus * 4;
         }
     }
     hipError_t e = hipMemcpyAsync(bt->segs_dev, bt->segs_host.data(),

@@ -92,8 +92,8 @@ class Beam:
 class Server:
     def __init__(self, hbm_budget=0, max_inflight=0, max_batch=0, persistent=False,
                  queue_depth=0):
-        cfg = ServerConfig(hbm_budget, max_inflight, max_batch, 1 if persistent else 0,
-                           queue_depth)
+        # persistent: False/0 off, True/1 default grid, n > 1: n workgroups
+        cfg = ServerConfig(hbm_budget, max_inflight, max_batch, int(persistent), queue_depth)
         h = C.c_void_p()
         check(_sig().mdsm_server_create(C.byref(cfg), C.byref(h)))
         self.h = h.value

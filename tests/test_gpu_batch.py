@@ -65,7 +65,8 @@ def test_batch_equals_plans_alone(gpu, persistent):
     plans = _plans(M)
     ref = _alone(M, plans)
     for p, _ in plans:
-        p.set_tuning(persistent=persistent, chunk_rows=4)
+        # a grid smaller than the work queue, so that persistent mode really pulls items
+        p.set_tuning(persistent=persistent, chunk_rows=4, grid=40 if persistent else 0)
     b = M.Batch([p for p, _ in plans])
     ins = [M.DeviceBuffer.from_host(x) for _, x in plans]
     outs = [M.DeviceBuffer(p.info.out_elems * 4) for p, _ in plans]
@@ -95,7 +96,7 @@ def test_batch_graph_replay_matches_eager(gpu):
     print("case batch graph replay", flush=True)
     plans = _plans(M, seed=8, n=7)
     for p, _ in plans:
-        p.set_tuning(persistent=1, chunk_rows=6)
+        p.set_tuning(persistent=1, chunk_rows=6, grid=40)
     b = M.Batch([p for p, _ in plans])
     ins = [M.DeviceBuffer.from_host(x) for _, x in plans]
     strides = [x.shape[1] for _, x in plans]

@@ -153,7 +153,7 @@ def test_persistent_launch_matches(gpu):
     C, N = 64, 4096
     xs = [_stream(M, C, N, 2, 50 + i, [(40.0, 1500, 8, 4.0)]) for i in range(3)]
     outs = []
-    for persistent in (False, True):
+    for persistent in (0, 24):  # 24 workgroups: fewer than the items, so the queue is used
         srv = Server(max_batch=8, persistent=persistent)
         bs = [srv.open_beam(**_beam_kwargs(C, N)) for _ in xs]
         r = _run(srv, list(zip(bs, xs)), N, 2)
