@@ -46,6 +46,8 @@ WORKLOADS = {
                               sub_dmstep=64 * (1500.0 / 4096), ncalls=64, calldms=64)]),
     # batched: 64 beams of uneven shapes, one launch per step, graph replay
     "cfg4": dict(beams=64, seed=4004),
+    # the other stages of the path, each at the size it has in configs 2 and 5
+    "stages": dict(),
     # serving: 32 beams of 1024-ch uint8 blocks of 2^16 samples, 2048 DMs over 0-1000
     "cfg5": dict(nchans=1024, nsamp=1 << 16, fch1=1500.0, foff=-300.0 / 1024, tsamp=64e-6,
                  dtype="u8", ranges=[(0.0, 1000.0 / 2048, 2048, 1)], beams=32,
@@ -186,6 +188,86 @@ def serve(M, w, args):
         "server_stats": {k: int(v) for k, v in st.items()},
     }
 
+
+
+def stages_bench(M, args):
+    """Every stage beside dedispersion, timed with the library's launch events
+    at the size it has in the configurations: corner turn and 2-bit unpack of a
+    config-2 block (1024 ch x 2^20), the fused uint8 decimation of config 2
+    (k = 2 and 4), and, on a config-5 block, the clipper (1024 ch x 65536 u8)
+    and the detectors and detrend on its DM x time output (2048 x 65536 f32).
+    Bytes are algorithmic (each input read once, each output written once)."""
+    reps = max(1, args.steps // 2)
+    res = {}
+
+    def timed(name, key, fn, nbytes, note):
+        fn()
+        M.synchronize()
+        M.timing_reset()
+        M.timing_enable(True)
+        for _ in range(reps):
+            fn()
+        M.synchronize()
+        ms, n = M.timing_get(key)
+        M.timing_enable(False)
+        per = ms / max(n, 1) * (n / reps)  # ms per call of fn
+        res[name] = {"ms": per, "bytes": nbytes, "gbs": nbytes / per / 1e6,
+                     "hbm_frac": nbytes / per / 1e6 / HBM_PEAK_GBS, "note": note}
+        print(f"[bench] stage {name}: {per:.3f} ms", file=sys.stderr, flush=True)
+
+    C, S = 1024, 1 << 20
+    x = M.synth_filterbank(C, S, "u8", seed=5, mean=128, sigma=16, nthreads=16)
+    tm = np.ascontiguousarray(x.T)
+    d_tm = M.DeviceBuffer.from_host(tm)
+    d_cm = M.DeviceBuffer(C * S)
+    timed("corner_turn_u8", "corner_turn",
+          lambda: M.corner_turn(d_tm, "u8", S, C, C, d_cm, S),
+          2 * C * S, "time-major [2^20][1024] u8 -> channel-major [1024][2^20] u8")
+    raw2 = np.random.default_rng(6).integers(0, 256, S * C // 4, dtype=np.uint8)
+    d_raw = M.DeviceBuffer.from_host(raw2)
+    timed("unpack_2bit", "unpack",
+          lambda: M.unpack_spectra(d_raw, 2, S, C, C // 4, d_cm, S, dtype_out="u8"),
+          C * S // 4 + C * S, "2-bit spectra [2^20][1024] -> channel-major u8 [1024][2^20]")
+    del d_tm, d_raw, tm, raw2
+    w2 = WORKLOADS["cfg2"]
+    plan = make_plan(M, w2)
+    d_in = M.DeviceBuffer.from_host(x)
+    d_out = M.DeviceBuffer(plan.info.out_elems * 4)
+    d_w = M.DeviceBuffer(max(plan.info.work_bytes, 256))
+    timed("decimate_u8_k2_k4", "decimate",
+          lambda: plan.dedisperse(d_in, S, d_out, d_w),
+          C * S + C * (S // 2) * 2 + C * (S // 4) * 2,
+          "config 2: uint8 -> uint16 2- and 4-sums in one pass (timed inside a dedispersion step)")
+    del d_in, d_out, d_w, plan
+    N5, D5 = 1 << 16, 2048
+    blk = np.ascontiguousarray(x[:, :N5])
+    d_blk = M.DeviceBuffer.from_host(blk)
+    d_clip = M.DeviceBuffer(C * N5)
+    timed("clip_u8", "clip",
+          lambda: M.clip_u8(d_blk, N5, C, 0, N5, 5.0, d_clip, N5),
+          2 * C * N5, "config-5 block: per-channel median/MAD clip, 1024 x 65536 u8")
+    y = np.random.default_rng(7).standard_normal((D5, N5), dtype=np.float32)
+    d_y = M.DeviceBuffer.from_host(y)
+    cap = 1 << 20
+    d_c, d_n = M.DeviceBuffer(cap * 16), M.DeviceBuffer(4)
+    d_mean, d_sig = M.DeviceBuffer(D5 * 8), M.DeviceBuffer(D5 * 8)
+    timed("boxcar_w1_64", "detect",
+          lambda: M.detect_boxcar(d_y, D5, N5, N5, [1, 2, 4, 8, 16, 32, 64], 6.0, d_c, cap, d_n,
+                                  d_mean, d_sig),
+          D5 * N5 * 4, "config-5 output 2048 x 65536 f32: row statistics + boxcar w = 1..64, S/N > 6")
+    dmv = M.DeviceBuffer.from_host(np.linspace(0.0, 1000.0, D5, dtype=np.float32))
+    d_st = M.DeviceBuffer((2 + 2 * D5) * 8)
+    timed("median5", "median5",
+          lambda: M.detect_median5(d_y, D5, N5, N5, dmv, 6.0, d_c, cap, d_n, d_st, per_row=True),
+          D5 * N5 * 4, "config-5 output: median-of-five detector, per-row statistics")
+    timed("detrend_seg1024", "detrend",
+          lambda: M.detrend(d_y, D5, N5, N5, 1024),
+          2 * D5 * N5 * 4, "config-5 output in place, 1024-sample segments")
+    return {"metric": "stage throughput (algorithmic bytes / kernel time)", "unit": "GB/s",
+            "n_gpus": 1, "data": "synthetic", "reps": reps,
+            "config": {"workload": "stages: corner turn, unpack, decimation at config 2; "
+                                   "clip, boxcar, median-of-five, detrend at config 5"},
+            "stages": res}
 
 
 def batch_bench(M, w, args):
@@ -427,6 +509,10 @@ def main():
         sys.exit(2)
     import mdsm_amd as M
     w = WORKLOADS[args.workload]
+    if args.workload == "stages":
+        print("[bench] stages", file=sys.stderr, flush=True)
+        print(json.dumps(stages_bench(M, args)), flush=True)
+        return
     if args.workload == "cfg4":
         print("[bench] cfg4 batched", file=sys.stderr, flush=True)
         print(json.dumps(batch_bench(M, w, args)), flush=True)

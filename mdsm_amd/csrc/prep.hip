@@ -42,6 +42,144 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// uint8 corner turn without LDS.  A lane owns 4 channels x 16 consecutive
+// spectra: it loads one dword (its 4 channels) of each of the 16 spectra,
+// transposes the four 4x4 byte blocks with v_perm_b32 and stores 16
+// consecutive samples of each channel with one 16-byte store.  A wave is CG
+// channel groups x (64/CG) spectrum groups, so a load instruction reads 4*CG
+// contiguous bytes of 64/CG spectra and a store writes 16*64/CG contiguous
+// bytes of CG channel rows; the workgroup's four waves take consecutive runs
+// of spectra.  Needs dword-aligned input rows, 16-byte-aligned output rows
+// and nchans % 4 == 0 (the host checks).
+__device__ __forceinline__ void transpose4x4(uint32_t a, uint32_t b, uint32_t c, uint32_t d,
+                                             uint32_t &p0, uint32_t &p1, uint32_t &p2, uint32_t &p3)
+{
+    // rows a..d hold channels 0..3 of samples 0..3; p_j = channel j, samples 0..3
+    const uint32_t t0 = __builtin_amdgcn_perm(b, a, 0x05010400u);  // a0 b0 a1 b1
+    const uint32_t t1 = __builtin_amdgcn_perm(b, a, 0x07030602u);  // a2 b2 a3 b3
+    const uint32_t t2 = __builtin_amdgcn_perm(d, c, 0x05010400u);  // c0 d0 c1 d1
+    const uint32_t t3 = __builtin_amdgcn_perm(d, c, 0x07030602u);  // c2 d2 c3 d3
+    p0 = __builtin_amdgcn_perm(t2, t0, 0x05040100u);  // a0 b0 c0 d0
+    p1 = __builtin_amdgcn_perm(t2, t0, 0x07060302u);  // a1 b1 c1 d1
+    p2 = __builtin_amdgcn_perm(t3, t1, 0x05040100u);
+    p3 = __builtin_amdgcn_perm(t3, t1, 0x07060302u);
+}
+
+#ifndef MDSM_CT_CG
+#define MDSM_CT_CG 16
+#endif
+#ifndef MDSM_CT_NT
+#define MDSM_CT_NT 16
+#endif
+constexpr int kCtCG = MDSM_CT_CG;     // channel groups (of 4) per wave
+constexpr int kCtNT = MDSM_CT_NT;     // spectra per lane (multiple of 16)
+constexpr int kCtTQ = 64 / kCtCG;     // spectrum groups per wave
+constexpr int kCtWaveT = kCtNT * kCtTQ;  // spectra per wave
+constexpr int kCtBlockT = 4 * kCtWaveT;
+constexpr int kCtBlockC = 4 * kCtCG;
+
+__global__ void __launch_bounds__(256)
+    corner_turn_u8_kernel(const uint8_t *__restrict__ in, int64_t in_stride, uint32_t nsamp,
+                          uint32_t nchans, int invert, uint8_t *__restrict__ out,
+                          int64_t out_stride, int64_t col0)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cg = lane % kCtCG, tq = lane / kCtCG;
+    const uint32_t cbase = blockIdx.y * (uint32_t)kCtBlockC + 4u * (uint32_t)cg;
+    const uint32_t tl = blockIdx.x * (uint32_t)kCtBlockT + (uint32_t)wave * kCtWaveT + kCtNT * tq;
+    if (cbase >= nchans || tl >= nsamp) return;  // nchans % 4 == 0: all 4 channels valid
+    // input column of the lane's channel group (read reversed when inverting)
+    const uint32_t icol = invert ? nchans - 4u - cbase : cbase;
+#pragma unroll
+    for (int h = 0; h < kCtNT / 16; ++h) {
+    const uint32_t t0 = tl + 16u * h;
+    if (t0 >= nsamp) break;
+    uint32_t v[16];
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const uint32_t t = t0 + i;
+        v[i] = t < nsamp ? *(const uint32_t *)(in + (int64_t)t * in_stride + icol) : 0u;
+        if (invert) v[i] = __builtin_amdgcn_perm(v[i], v[i], 0x00010203u);  // reverse bytes
+    }
+    uint32_t q[4][4];  // q[j][k]: channel j, samples 4k..4k+3
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        transpose4x4(v[4 * k], v[4 * k + 1], v[4 * k + 2], v[4 * k + 3], q[0][k], q[1][k],
+                     q[2][k], q[3][k]);
+    const bool full = t0 + 16u <= nsamp;
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+        uint8_t *o = out + (int64_t)(cbase + j) * out_stride + col0 + t0;
+        if (full) {
+            *(uint4 *)o = make_uint4(q[j][0], q[j][1], q[j][2], q[j][3]);
+        } else {
+            for (uint32_t i = 0; i < 16u && t0 + i < nsamp; ++i)
+                o[i] = (uint8_t)(q[j][i >> 2] >> (8 * (i & 3)));
+        }
+    }
+    }
+}
+
+// Wide variant: a lane owns 16 channels x 16 spectra (16-byte loads and
+// stores, a 16x16 byte transpose as sixteen 4x4 blocks).  A wave is 8
+// channel groups x 8 spectrum groups: a load instruction reads 128
+// contiguous bytes of 8 spectra, a store writes 128 contiguous bytes of 8
+// channel rows.  Needs 16-byte-aligned input and output rows and
+// nchans % 16 == 0.
+constexpr int kCw = 8;                  // 16-channel groups per wave
+constexpr int kCwT = 16 * (64 / kCw);   // spectra per wave
+constexpr int kCwBlockT = 4 * kCwT, kCwBlockC = 16 * kCw;
+
+__global__ void __launch_bounds__(256)
+    corner_turn_u8x16_kernel(const uint8_t *__restrict__ in, int64_t in_stride, uint32_t nsamp,
+                             uint32_t nchans, int invert, uint8_t *__restrict__ out,
+                             int64_t out_stride, int64_t col0)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int cg = lane % kCw, tq = lane / kCw;
+    const uint32_t cbase = blockIdx.y * (uint32_t)kCwBlockC + 16u * (uint32_t)cg;
+    const uint32_t t0 = blockIdx.x * (uint32_t)kCwBlockT + (uint32_t)wave * kCwT + 16u * tq;
+    if (cbase >= nchans || t0 >= nsamp) return;  // nchans % 16 == 0: all 16 channels valid
+    const uint32_t icol = invert ? nchans - 16u - cbase : cbase;
+    uint32_t v[16][4];  // v[i][g]: spectrum t0+i, channels 4g..4g+3 of the lane's 16
+#pragma unroll
+    for (int i = 0; i < 16; ++i) {
+        const uint32_t t = t0 + i;
+        uint4 w = make_uint4(0u, 0u, 0u, 0u);
+        if (t < nsamp) w = *(const uint4 *)(in + (int64_t)t * in_stride + icol);
+        if (invert) {  // reverse the 16 bytes
+            v[i][0] = __builtin_amdgcn_perm(w.w, w.w, 0x00010203u);
+            v[i][1] = __builtin_amdgcn_perm(w.z, w.z, 0x00010203u);
+            v[i][2] = __builtin_amdgcn_perm(w.y, w.y, 0x00010203u);
+            v[i][3] = __builtin_amdgcn_perm(w.x, w.x, 0x00010203u);
+        } else {
+            v[i][0] = w.x;
+            v[i][1] = w.y;
+            v[i][2] = w.z;
+            v[i][3] = w.w;
+        }
+    }
+    const bool full = t0 + 16u <= nsamp;
+#pragma unroll
+    for (int g = 0; g < 4; ++g) {
+        uint32_t q[4][4];  // q[j][k]: channel 4g+j, samples 4k..4k+3
+#pragma unroll
+        for (int k = 0; k < 4; ++k)
+            transpose4x4(v[4 * k][g], v[4 * k + 1][g], v[4 * k + 2][g], v[4 * k + 3][g], q[0][k],
+                         q[1][k], q[2][k], q[3][k]);
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            uint8_t *o = out + (int64_t)(cbase + 4 * g + j) * out_stride + col0 + t0;
+            if (full) {
+                *(uint4 *)o = make_uint4(q[j][0], q[j][1], q[j][2], q[j][3]);
+            } else {
+                for (uint32_t i = 0; i < 16u && t0 + i < nsamp; ++i)
+                    o[i] = (uint8_t)(q[j][i >> 2] >> (8 * (i & 3)));
+            }
+        }
+    }
+}
+
 // Packed SIGPROC spectra: sample c of a spectrum starts at bit c*NB, least
 // significant bits first within a byte (NB <= 8); 16-bit unsigned and 32-bit
 // float samples are plain arrays.
@@ -163,6 +301,30 @@ extern "C" int mdsm_corner_turn(const void *d_in, int32_t dtype_in, uint32_t nsa
     if (nchans > 65535u * kCT) return fail(MDSM_ERR_INVALID, "nchans too large");
     hipStream_t s = (hipStream_t)stream;
     TimedLaunch tl("corner_turn", s);
+    if (dtype_in == MDSM_U8 && dtype_out == MDSM_U8 && nchans % 16 == 0 &&
+        (uintptr_t)d_in % 16 == 0 && in_stride % 16 == 0 &&
+        ((uintptr_t)d_out + (uint64_t)out_col0) % 16 == 0 && out_stride % 16 == 0 &&
+        (nchans + kCwBlockC - 1) / kCwBlockC <= 65535u) {
+        dim3 g16((nsamp + kCwBlockT - 1) / kCwBlockT, (nchans + kCwBlockC - 1) / kCwBlockC);
+        hipLaunchKernelGGL(corner_turn_u8x16_kernel, g16, dim3(256), 0, s, (const uint8_t *)d_in,
+                           in_stride, nsamp, nchans, invert ? 1 : 0, (uint8_t *)d_out, out_stride,
+                           out_col0);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "corner turn launch");
+        return MDSM_OK;
+    }
+    if (dtype_in == MDSM_U8 && dtype_out == MDSM_U8 && nchans % 4 == 0 &&
+        (uintptr_t)d_in % 4 == 0 && in_stride % 4 == 0 &&
+        ((uintptr_t)d_out + (uint64_t)out_col0) % 16 == 0 && out_stride % 16 == 0 &&
+        (nchans + kCtBlockC - 1) / kCtBlockC <= 65535u) {
+        dim3 g8((nsamp + kCtBlockT - 1) / kCtBlockT, (nchans + kCtBlockC - 1) / kCtBlockC);
+        hipLaunchKernelGGL(corner_turn_u8_kernel, g8, dim3(256), 0, s, (const uint8_t *)d_in,
+                           in_stride, nsamp, nchans, invert ? 1 : 0, (uint8_t *)d_out, out_stride,
+                           out_col0);
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "corner turn launch");
+        return MDSM_OK;
+    }
     dim3 grid((nsamp + kCT - 1) / kCT, (nchans + kCT - 1) / kCT);
     const int inv = invert ? 1 : 0;
 #define MDSM_CT(TI, TO)                                                                        \

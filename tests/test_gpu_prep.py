@@ -35,6 +35,32 @@ def test_corner_turn_matches_oracle(gpu, dtype, out, invert):
     assert np.all(got[:, :col0].view(np.uint8) == 0xFF)
 
 
+@pytest.mark.parametrize("invert", [False, True])
+@pytest.mark.parametrize("T,Cn,pad", [(1000, 300, 8), (4096, 1024, 8), (17, 4, 8),
+                                      (1000, 320, 16), (4096, 1024, 0), (33, 16, 16)])
+def test_corner_turn_u8_aligned_ragged(gpu, T, Cn, pad, invert):
+    """uint8 rows with aligned input and 16-byte-aligned output (the
+    register-transpose kernels: dword rows with nchans % 4 == 0, 16-byte rows
+    with nchans % 16 == 0): spectra counts off the 16-sample and workgroup
+    grids, channel counts off the channel blocks; bytes outside the window
+    untouched."""
+    M = gpu
+    print(f"case corner_turn_u8_aligned T={T} C={Cn} pad={pad} invert={invert}", flush=True)
+    rng = np.random.default_rng(14)
+    in_stride, col0 = Cn + pad, 16
+    out_stride = (col0 + T + 15) // 16 * 16 + 32
+    x = rng.integers(0, 256, (T, in_stride), dtype=np.uint8)
+    din = M.DeviceBuffer.from_host(x)
+    dout = M.DeviceBuffer(Cn * out_stride)
+    dout.fill_u32(0xFFFFFFFF)
+    M.corner_turn(din, "u8", T, Cn, in_stride, dout, out_stride, col0, invert, "u8")
+    M.synchronize()
+    got = dout.download(np.uint8, (Cn, out_stride))
+    exp = x[:, :Cn][:, ::-1].T if invert else x[:, :Cn].T
+    assert np.array_equal(got[:, col0:col0 + T], exp)
+    assert np.all(got[:, :col0] == 0xFF) and np.all(got[:, col0 + T:] == 0xFF)
+
+
 def test_corner_turn_full_config2_block(gpu):
     """1024 channels x 2^20 uint8 samples: bytes round-trip exactly (checksum of
     the transposed block equals the oracle's on a slice and in total)."""
