@@ -120,16 +120,52 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// Wide variant: a lane owns 16 channels x 16 spectra (16-byte loads and
-// stores, a 16x16 byte transpose as sixteen 4x4 blocks).  A wave is 8
-// channel groups x 8 spectrum groups: a load instruction reads 128
-// contiguous bytes of 8 spectra, a store writes 128 contiguous bytes of 8
-// channel rows.  Needs 16-byte-aligned input and output rows and
+// Wide variant: a lane owns 16 channels x 16 spectra (16-byte stores, a
+// 16x16 byte transpose as sixteen 4x4 blocks).  A wave is 8 channel groups x
+// 8 spectrum groups: a store writes 128 contiguous bytes of 8 channel rows,
+// a load reads 16*NB contiguous bytes of 8 spectra.  NB = 8 is the uint8
+// corner turn; NB = 1, 2, 4 unpack packed SIGPROC spectra on the way.  Needs
+// input rows aligned to 2*NB bytes, 16-byte-aligned output rows and
 // nchans % 16 == 0.
 constexpr int kCw = 8;                  // 16-channel groups per wave
 constexpr int kCwT = 16 * (64 / kCw);   // spectra per wave
 constexpr int kCwBlockT = 4 * kCwT, kCwBlockC = 16 * kCw;
 
+// 16 consecutive samples of one spectrum as 16 bytes (channel order),
+// from NB-bit packed samples (least significant bits first within a byte):
+// 2*NB bytes at p, aligned to 2*NB.
+template <int NB>
+__device__ __forceinline__ uint4 load16(const uint8_t *p)
+{
+    if constexpr (NB == 8) {
+        return *(const uint4 *)p;
+    } else if constexpr (NB == 4) {
+        const uint2 u = *(const uint2 *)p;
+        auto spread = [](uint32_t x) {  // 2 bytes of nibbles -> 4 bytes
+            const uint32_t y = (x & 0xffu) | ((x & 0xff00u) << 8);
+            return (y | (y << 4)) & 0x0f0f0f0fu;
+        };
+        return make_uint4(spread(u.x & 0xffffu), spread(u.x >> 16), spread(u.y & 0xffffu),
+                          spread(u.y >> 16));
+    } else if constexpr (NB == 2) {
+        const uint32_t d = *(const uint32_t *)p;
+        auto spread = [](uint32_t b) {  // 1 byte of 2-bit samples -> 4 bytes
+            return (b | (b << 6) | (b << 12) | (b << 18)) & 0x03030303u;
+        };
+        return make_uint4(spread(d & 0xffu), spread((d >> 8) & 0xffu), spread((d >> 16) & 0xffu),
+                          spread(d >> 24));
+    } else {
+        static_assert(NB == 1, "1, 2, 4 or 8 bits");
+        const uint32_t h = *(const uint16_t *)p;
+        auto spread = [](uint32_t n) {  // 4 bits -> 4 bytes
+            return (n | (n << 7) | (n << 14) | (n << 21)) & 0x01010101u;
+        };
+        return make_uint4(spread(h & 0xfu), spread((h >> 4) & 0xfu), spread((h >> 8) & 0xfu),
+                          spread(h >> 12));
+    }
+}
+
+template <int NB>
 __global__ void __launch_bounds__(256)
     corner_turn_u8x16_kernel(const uint8_t *__restrict__ in, int64_t in_stride, uint32_t nsamp,
                              uint32_t nchans, int invert, uint8_t *__restrict__ out,
@@ -146,7 +182,7 @@ __global__ void __launch_bounds__(256)
     for (int i = 0; i < 16; ++i) {
         const uint32_t t = t0 + i;
         uint4 w = make_uint4(0u, 0u, 0u, 0u);
-        if (t < nsamp) w = *(const uint4 *)(in + (int64_t)t * in_stride + icol);
+        if (t < nsamp) w = load16<NB>(in + (int64_t)t * in_stride + icol * NB / 8);
         if (invert) {  // reverse the 16 bytes
             v[i][0] = __builtin_amdgcn_perm(w.w, w.w, 0x00010203u);
             v[i][1] = __builtin_amdgcn_perm(w.z, w.z, 0x00010203u);
@@ -306,7 +342,7 @@ extern "C" int mdsm_corner_turn(const void *d_in, int32_t dtype_in, uint32_t nsa
         ((uintptr_t)d_out + (uint64_t)out_col0) % 16 == 0 && out_stride % 16 == 0 &&
         (nchans + kCwBlockC - 1) / kCwBlockC <= 65535u) {
         dim3 g16((nsamp + kCwBlockT - 1) / kCwBlockT, (nchans + kCwBlockC - 1) / kCwBlockC);
-        hipLaunchKernelGGL(corner_turn_u8x16_kernel, g16, dim3(256), 0, s, (const uint8_t *)d_in,
+        hipLaunchKernelGGL(corner_turn_u8x16_kernel<8>, g16, dim3(256), 0, s, (const uint8_t *)d_in,
                            in_stride, nsamp, nchans, invert ? 1 : 0, (uint8_t *)d_out, out_stride,
                            out_col0);
         hipError_t e = hipGetLastError();
@@ -381,6 +417,24 @@ extern "C" int mdsm_unpack_spectra(const void *d_raw, int32_t nbits, uint32_t ns
     if (nchans > 65535u * kCT) return fail(MDSM_ERR_INVALID, "nchans too large");
     hipStream_t s = (hipStream_t)stream;
     TimedLaunch tl("unpack", s);
+    if (dtype_out == MDSM_U8 && nbits <= 8 && nchans % 16 == 0 &&
+        (uintptr_t)d_raw % (2 * nbits) == 0 && in_stride_bytes % (2 * nbits) == 0 &&
+        ((uintptr_t)d_out + (uint64_t)out_col0) % 16 == 0 && out_stride % 16 == 0 &&
+        (nchans + kCwBlockC - 1) / kCwBlockC <= 65535u) {
+        dim3 g16((nsamp + kCwBlockT - 1) / kCwBlockT, (nchans + kCwBlockC - 1) / kCwBlockC);
+        const uint8_t *r = (const uint8_t *)d_raw;
+        uint8_t *o = (uint8_t *)d_out;
+        const int inv = invert ? 1 : 0;
+        switch (nbits) {
+        case 1: hipLaunchKernelGGL(corner_turn_u8x16_kernel<1>, g16, dim3(256), 0, s, r, in_stride_bytes, nsamp, nchans, inv, o, out_stride, out_col0); break;
+        case 2: hipLaunchKernelGGL(corner_turn_u8x16_kernel<2>, g16, dim3(256), 0, s, r, in_stride_bytes, nsamp, nchans, inv, o, out_stride, out_col0); break;
+        case 4: hipLaunchKernelGGL(corner_turn_u8x16_kernel<4>, g16, dim3(256), 0, s, r, in_stride_bytes, nsamp, nchans, inv, o, out_stride, out_col0); break;
+        default: hipLaunchKernelGGL(corner_turn_u8x16_kernel<8>, g16, dim3(256), 0, s, r, in_stride_bytes, nsamp, nchans, inv, o, out_stride, out_col0); break;
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return hip_fail(e, "unpack launch");
+        return MDSM_OK;
+    }
     dim3 grid((nsamp + kCT - 1) / kCT, (nchans + kCT - 1) / kCT);
     const int inv = invert ? 1 : 0;
     const uint8_t *raw = (const uint8_t *)d_raw;

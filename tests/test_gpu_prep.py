@@ -101,11 +101,14 @@ def test_detrend_matches_oracle(gpu, n, seg):
     assert np.array_equal(got[:, n:], x[:, n:])   # padding untouched
 
 
+@pytest.mark.parametrize("aligned", [False, True])
 @pytest.mark.parametrize("nbits,out", [(1, "u8"), (2, "u8"), (4, "u8"), (8, "u8"), (1, "f32"),
                                        (4, "f32"), (8, "f32"), (16, "f32"), (32, "f32")])
-def test_unpack_spectra_matches_oracle(gpu, nbits, out):
+def test_unpack_spectra_matches_oracle(gpu, nbits, out, aligned):
+    """aligned: 16-byte-aligned output rows (the register-transpose kernel
+    for uint8 output); otherwise the tiled kernel."""
     M = gpu
-    print(f"case unpack nbits={nbits} -> {out}", flush=True)
+    print(f"case unpack nbits={nbits} -> {out} aligned={aligned}", flush=True)
     rng = np.random.default_rng(20 + nbits)
     T, Cn = 700, 192
     sb = Cn * nbits // 8
@@ -117,7 +120,7 @@ def test_unpack_spectra_matches_oracle(gpu, nbits, out):
     exp = np.stack([oracle.unpack_bits(raw[t, :sb].tobytes(), nbits, Cn) for t in range(T)])
     for invert in (False, True):
         d_raw = M.DeviceBuffer.from_host(raw)
-        col0, ostride = 5, T + 9
+        col0, ostride = (16, 736) if aligned else (5, T + 9)
         npd = np.uint8 if out == "u8" else np.float32
         d_out = M.DeviceBuffer(Cn * ostride * np.dtype(npd).itemsize)
         M.unpack_spectra(d_raw, nbits, T, Cn, stride, d_out, ostride, col0, invert, out)
