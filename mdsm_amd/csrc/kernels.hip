@@ -1088,15 +1088,29 @@ __global__ void __launch_bounds__(256)
     }
 }
 
-// Register form for widths up to 64: a wave holds 256 consecutive samples of
-// a row (4 per lane, starting one before its tile) and builds S_2w(t) =
-// S_w(t) + S_w(t+w) in registers with cross-lane shifts, the same tree and
-// FP64 arithmetic as above; each wave reports the kRT tile positions whose
-// widest sum and right neighbour lie inside its 256 samples.
-constexpr int kRT = 188;
+// Register form for widths up to 64: a wave holds kBW = 1024 consecutive
+// samples of a row (16 per lane, starting 4 before its tile) and builds
+// S_2w(t) = S_w(t) + S_w(t+w) in registers -- the same tree and FP64
+// arithmetic as above.  Partners of levels w < 16 are mostly in the lane; the
+// rest come from the next lane by a DPP wave shift (wave_shl:1, lane i reads
+// lane i+1), level 16 takes the next lane's whole set the same way and only
+// level 32 needs a two-lane shuffle.  Each wave reports the kBR tile
+// positions whose widest sum and right neighbour lie inside its window.
+constexpr int kBL = 16;                 // samples per lane
+constexpr int kBW = 64 * kBL;           // samples per wave
+constexpr int kBR = kBW - 4 - 64;       // positions reported per wave (maxw <= 64)
 
 __device__ __forceinline__ double shfl_down_d(double v, int k) { return __shfl_down(v, k, 64); }
 __device__ __forceinline__ double shfl_up_d(double v, int k) { return __shfl_up(v, k, 64); }
+// v of lane i+1 (lane 63 keeps its own): DPP wave_shl:1 on both halves
+__device__ __forceinline__ double next_lane_d(double v)
+{
+    const uint64_t u = __builtin_bit_cast(uint64_t, v);
+    const int lo = (int)(uint32_t)u, hi = (int)(uint32_t)(u >> 32);
+    const uint32_t nlo = (uint32_t)__builtin_amdgcn_update_dpp(lo, lo, 0x130, 0xf, 0xf, false);
+    const uint32_t nhi = (uint32_t)__builtin_amdgcn_update_dpp(hi, hi, 0x130, 0xf, 0xf, false);
+    return __builtin_bit_cast(double, ((uint64_t)nhi << 32) | nlo);
+}
 
 __global__ void __launch_bounds__(256)
     boxcar_reg_kernel(const float *x, uint32_t n, int64_t stride, uint32_t wmask, uint32_t maxw,
@@ -1109,44 +1123,59 @@ __global__ void __launch_bounds__(256)
     const double mu = mean[d];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
-    const int64_t T0 = ((int64_t)blockIdx.x * 4 + wave) * kRT;
+    const int64_t T0 = ((int64_t)blockIdx.x * 4 + wave) * kBR;
     if (T0 >= (int64_t)n) return;  // the whole wave
     const float *row = x + (int64_t)d * stride;
-    const int64_t b = T0 - 1;       // sample held by lane 0, slot 0
-    double s[4];
+    const int64_t b = T0 - 4;       // sample held by lane 0, slot 0
+    const int64_t tl = b + kBL * lane;
+    double s[kBL];
+    if (tl >= 0 && tl + kBL <= (int64_t)n && (((uintptr_t)(row + tl)) & 15) == 0) {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-        const int64_t t = b + 4 * lane + j;
-        s[j] = (t >= 0 && t < (int64_t)n) ? (double)row[t] : 0.0;
+        for (int q = 0; q < kBL / 4; ++q) {
+            const float4 v = ((const float4 *)(row + tl))[q];
+            s[4 * q] = v.x;
+            s[4 * q + 1] = v.y;
+            s[4 * q + 2] = v.z;
+            s[4 * q + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int j = 0; j < kBL; ++j) {
+            const int64_t t = tl + j;
+            s[j] = (t >= 0 && t < (int64_t)n) ? (double)row[t] : 0.0;
+        }
     }
-    for (uint32_t w = 1, wi = 0; w <= maxw; w <<= 1, ++wi) {
+#pragma unroll
+    for (int lv = 0; lv < 7; ++lv) {
+        const uint32_t w = 1u << lv;
+        if (w > maxw) break;
         if (wmask & w) {
             const double wm = (double)w * mu;
-            const double den = sg * sqrtw.v[wi];
+            const double den = sg * sqrtw.v[lv];
             const double td = thr * den;
             const double lo = td - fabs(td) * 0x1p-40;  // exact pre-filter, as above
             const int64_t last = (int64_t)n - w;
-            bool c[4];
+            bool c[kBL];
             bool any = false;
 #pragma unroll
-            for (int j = 0; j < 4; ++j) {
-                const int u = 4 * lane + j;
+            for (int j = 0; j < kBL; ++j) {
+                const int u = kBL * lane + j;
                 const int64_t t = b + u;
-                c[j] = u >= 1 && u <= kRT && t <= last && (s[j] - wm) > lo;
+                c[j] = u >= 4 && u < 4 + kBR && t <= last && (s[j] - wm) > lo;
                 any = any || c[j];
             }
             if (__any(any)) {
-                const double prev3 = shfl_up_d(s[3], 1);
-                const double next0 = shfl_down_d(s[0], 1);
+                const double prevL = shfl_up_d(s[kBL - 1], 1);
+                const double nextF = next_lane_d(s[0]);
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
+                for (int j = 0; j < kBL; ++j) {
                     if (!c[j]) continue;
-                    const int64_t t = b + 4 * lane + j;
+                    const int64_t t = b + kBL * lane + j;
                     const double sv = s[j];
                     const double snr = (sv - wm) / den;
                     if (!(snr > thr)) continue;
-                    const double pv = j > 0 ? s[j > 0 ? j - 1 : 0] : prev3;
-                    const double nv = j < 3 ? s[j < 3 ? j + 1 : 3] : next0;
+                    const double pv = j > 0 ? s[j > 0 ? j - 1 : 0] : prevL;
+                    const double nv = j < kBL - 1 ? s[j < kBL - 1 ? j + 1 : 0] : nextF;
                     if (t > 0 && !(sv >= pv)) continue;
                     if (t < last && !(sv > nv)) continue;
                     const uint32_t slot = atomicAdd(count, 1u);
@@ -1162,22 +1191,22 @@ __global__ void __launch_bounds__(256)
             }
         }
         if (2 * w > maxw) break;
-        if (w == 1) {
-            const double n0 = shfl_down_d(s[0], 1);
-            s[0] += s[1];
-            s[1] += s[2];
-            s[2] += s[3];
-            s[3] += n0;
-        } else if (w == 2) {
-            const double n0 = shfl_down_d(s[0], 1), n1 = shfl_down_d(s[1], 1);
-            s[0] += s[2];
-            s[1] += s[3];
-            s[2] += n0;
-            s[3] += n1;
-        } else {
-            const int k = (int)(w >> 2);
+        // next level, S_2w(t) = S_w(t) + S_w(t + w), reading only old values
+        if (w < (uint32_t)kBL) {
+            double nx[kBL];
 #pragma unroll
-            for (int j = 0; j < 4; ++j) s[j] += shfl_down_d(s[j], k);
+            for (int j = 0; j < kBL; ++j)
+                if (j < (int)w) nx[j] = next_lane_d(s[j]);
+#pragma unroll
+            for (int j = 0; j < kBL; ++j)
+                s[j] += (j + (int)w < kBL) ? s[(j + (int)w) % kBL] : nx[(j + (int)w) % kBL];
+        } else if (w == (uint32_t)kBL) {
+#pragma unroll
+            for (int j = 0; j < kBL; ++j) s[j] += next_lane_d(s[j]);
+        } else {
+            const int k = (int)(w / kBL);
+#pragma unroll
+            for (int j = 0; j < kBL; ++j) s[j] += shfl_down_d(s[j], k);
         }
     }
 }
@@ -1194,7 +1223,7 @@ hipError_t launch_boxcar(const float *x, uint32_t ndms, uint32_t n, int64_t stri
                        sigma);
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, count, 1u, 0u);
     if (maxw <= 64) {
-        const uint32_t tiles = (n + kRT - 1) / kRT;
+        const uint32_t tiles = (n + kBR - 1) / kBR;
         hipLaunchKernelGGL(boxcar_reg_kernel, dim3((tiles + 3) / 4, ndms), dim3(256), 0, s, x, n,
                            stride, wmask, maxw, thr, sqrtw, cands, cap, count, mean, sigma);
         return hipGetLastError();
