@@ -316,6 +316,57 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Segments of at most 1024 samples: one wave per segment, the samples held
+// in registers (16 per lane, lane-strided so every load and store is 256
+// contiguous bytes per wave), FP64 sums by wave reductions: the segment is
+// read once and written once.
+constexpr int kDtK = 16;  // samples per lane
+
+__global__ void __launch_bounds__(256)
+    detrend_wave_kernel(float *x, int64_t stride, uint32_t n, uint32_t seg_len, uint32_t nseg)
+{
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const uint32_t seg = blockIdx.x * 4u + (uint32_t)wave;
+    if (seg >= nseg) return;
+    const uint32_t row = blockIdx.y;
+    const uint32_t s0 = seg * seg_len;
+    const uint32_t L = min(seg_len, n - s0);
+    float *p = x + (int64_t)row * stride + s0;
+    const double xc = -(double)L / 2.0 + 0.5;  // abscissa of sample 0
+    float y[kDtK];
+    double sy = 0.0, sxy = 0.0;
+#pragma unroll
+    for (int k = 0; k < kDtK; ++k) {
+        const uint32_t i = (uint32_t)lane + 64u * k;
+        y[k] = i < L ? p[i] : 0.0f;
+        const double v = y[k];
+        sy += v;
+        sxy += (xc + (double)i) * v;  // zero where i >= L
+    }
+    sy = wave_sum_d(sy);
+    sxy = wave_sum_d(sxy);
+    const double sxx = (double)L * ((double)L * (double)L - 1.0) / 12.0;
+    const double a = sy / (double)L;
+    const double b = sxx > 0.0 ? sxy / sxx : 0.0;
+    double ss = 0.0;
+#pragma unroll
+    for (int k = 0; k < kDtK; ++k) {
+        const uint32_t i = (uint32_t)lane + 64u * k;
+        const double r = (double)y[k] - (a + b * (xc + (double)i));
+        if (i < L) ss += r * r;
+    }
+    ss = wave_sum_d(ss);
+    const double rms = sqrt(ss / (double)L);
+#pragma unroll
+    for (int k = 0; k < kDtK; ++k) {
+        const uint32_t i = (uint32_t)lane + 64u * k;
+        if (i < L) {
+            const double r = (double)y[k] - (a + b * (xc + (double)i));
+            p[i] = rms > 0.0 ? (float)(r / rms) : 0.0f;
+        }
+    }
+}
+
 }  // namespace
 }  // namespace mdsm
 
@@ -392,8 +443,14 @@ extern "C" int mdsm_detrend(float *d_x, uint32_t nrows, uint32_t n, int64_t stri
     if (nrows > 65535) return fail(MDSM_ERR_INVALID, "nrows > 65535");
     hipStream_t s = (hipStream_t)stream;
     TimedLaunch tl("detrend", s);
-    dim3 grid((n + seg_len - 1) / seg_len, nrows);
-    hipLaunchKernelGGL(detrend_kernel, grid, dim3(256), 0, s, d_x, stride, n, seg_len);
+    const uint32_t nseg = (uint32_t)(((uint64_t)n + seg_len - 1) / seg_len);
+    if (seg_len <= 64u * kDtK) {
+        dim3 g((nseg + 3) / 4, nrows);
+        hipLaunchKernelGGL(detrend_wave_kernel, g, dim3(256), 0, s, d_x, stride, n, seg_len, nseg);
+    } else {
+        dim3 grid(nseg, nrows);
+        hipLaunchKernelGGL(detrend_kernel, grid, dim3(256), 0, s, d_x, stride, n, seg_len);
+    }
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_fail(e, "detrend launch");
     return MDSM_OK;

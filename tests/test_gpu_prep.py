@@ -78,7 +78,7 @@ def test_corner_turn_full_config2_block(gpu):
     assert np.array_equal(got.sum(axis=1, dtype=np.uint64), x.sum(axis=0, dtype=np.uint64)[::-1])
 
 
-@pytest.mark.parametrize("n,seg", [(4096, 1024), (5000, 1024), (777, 64)])
+@pytest.mark.parametrize("n,seg", [(4096, 1024), (5000, 1024), (777, 64), (3001, 1000), (5000, 2000)])
 def test_detrend_matches_oracle(gpu, n, seg):
     M = gpu
     print(f"case detrend n={n} seg={seg}", flush=True)
