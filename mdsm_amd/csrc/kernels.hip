@@ -925,10 +925,23 @@ __global__ void __launch_bounds__(256)
     uint8_t *o = out + c * out_stride + t0;
     const uint32_t rank = (n - 1) / 2;
 
+    // 16-byte loads and stores when both rows allow them (uniform per
+    // workgroup); the remaining n % 16 samples one at a time
+    const bool vec = ((uintptr_t)x % 16 == 0) && ((uintptr_t)o % 16 == 0);
+    const uint32_t nv = vec ? n / 16u : 0u;
+    auto for_each = [&](auto &&fn) {  // fn(sample index, value) over [0, n)
+        for (uint32_t q = tid; q < nv; q += 256) {
+            const uint4 w = ((const uint4 *)x)[q];
+            const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+            for (int k = 0; k < 16; ++k) fn(16u * q + k, (int)((ws[k >> 2] >> (8 * (k & 3))) & 0xffu));
+        }
+        for (uint32_t t = 16u * nv + tid; t < n; t += 256) fn(t, (int)x[t]);
+    };
     hist[tid] = 0;
     if (tid == 0) sh_cnt = 0;
     __syncthreads();
-    for (uint32_t t = tid; t < n; t += 256) atomicAdd(&hist[x[t]], 1u);
+    for_each([&](uint32_t, int v) { atomicAdd(&hist[v], 1u); });
     __syncthreads();
     if (tid < 64) {
         const int m = hist_rank(hist, rank, lane);
@@ -938,7 +951,7 @@ __global__ void __launch_bounds__(256)
     const int med = sh_med;
     hist[tid] = 0;
     __syncthreads();
-    for (uint32_t t = tid; t < n; t += 256) atomicAdd(&hist[abs((int)x[t] - med)], 1u);
+    for_each([&](uint32_t, int v) { atomicAdd(&hist[abs(v - med)], 1u); });
     __syncthreads();
     if (tid < 64) {
         const int m = hist_rank(hist, rank, lane);
@@ -948,7 +961,21 @@ __global__ void __launch_bounds__(256)
     const int mad = sh_mad;
     const float thr = (nsigma * 1.4826f) * (float)mad;
     uint32_t mine = 0;
-    for (uint32_t t = tid; t < n; t += 256) {
+    for (uint32_t q = tid; q < nv; q += 256) {
+        const uint4 w = ((const uint4 *)x)[q];
+        uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 16; ++k) {
+            const int sh = 8 * (k & 3);
+            const int v = (int)((ws[k >> 2] >> sh) & 0xffu);
+            if ((float)abs(v - med) > thr) {
+                ++mine;
+                ws[k >> 2] = (ws[k >> 2] & ~(0xffu << sh)) | ((uint32_t)med << sh);
+            }
+        }
+        ((uint4 *)o)[q] = make_uint4(ws[0], ws[1], ws[2], ws[3]);
+    }
+    for (uint32_t t = 16u * nv + tid; t < n; t += 256) {
         const int v = x[t];
         const bool clip = (float)abs(v - med) > thr;
         mine += clip;

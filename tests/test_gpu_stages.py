@@ -60,10 +60,13 @@ def test_subband_one_channel_per_subband_equals_brute(gpu):
 
 
 # ---------------------------------------------------------------- clipper
-def test_clip_u8_matches_oracle(gpu):
+@pytest.mark.parametrize("S,t0,n", [(70000, 1234, 65536), (65568, 16, 65531), (65536, 0, 65536)])
+def test_clip_u8_matches_oracle(gpu, S, t0, n):
+    """Unaligned rows (per-sample passes) and 16-byte-aligned rows (vector
+    passes, with and without a ragged tail)."""
     M = gpu
-    C, S, t0, n = 64, 70000, 1234, 65536
-    print("case clip_u8", flush=True)
+    C = 64
+    print(f"case clip_u8 S={S} t0={t0} n={n}", flush=True)
     x = M.synth_filterbank(C, S, "u8", seed=41, mean=128, sigma=16)
     rng = np.random.default_rng(41)
     x[3, rng.integers(t0, t0 + n, 200)] = 255          # impulsive RFI
