@@ -1323,33 +1323,50 @@ __global__ void __launch_bounds__(256)
                    mdsm_cand *cands, uint32_t cap, uint32_t *count, const double *stats,
                    uint32_t ndms)
 {
+    // a thread takes 4 consecutive outputs l0..l0+3 from samples l0..l0+7
     const uint32_t d = blockIdx.y;
-    const uint32_t l = blockIdx.x * 256u + threadIdx.x;
-    if (n < 5 || l >= n - 4) return;
+    const uint32_t l0 = 4u * (blockIdx.x * 256u + threadIdx.x);
+    if (n < 5 || l0 >= n - 4) return;
     if (!(dmv[d] > 1.0f)) return;
-    float lm, lr;
+    float rm = 0.0f, rr = 0.0f;  // row statistics (modes 0 and 1)
     if (mode == 0) {
-        lm = (float)stats[0];
-        lr = (float)stats[1];
+        rm = (float)stats[0];
+        rr = (float)stats[1];
     } else if (mode == 1) {
-        lm = (float)(stats[2 + d] / (double)n);
-        const double v = stats[2 + ndms + d] / (double)n - (double)lm * (double)lm;
-        lr = (float)sqrt(v);
-    } else {
-        lm = cmean[l / spc];
-        lr = crms[l / spc];
+        rm = (float)(stats[2 + d] / (double)n);
+        const double v = stats[2 + ndms + d] / (double)n - (double)rm * (double)rm;
+        rr = (float)sqrt(v);
     }
-    const float *r = x + (int64_t)d * stride + l;
-    const float m = med5(r[0], r[1], r[2], r[3], r[4]);
-    if (m - lm >= lr * thr) {
-        const uint32_t slot = atomicAdd(count, 1u);
-        if (slot < cap) {
-            mdsm_cand cd;
-            cd.dm = d;
-            cd.t = l;
-            cd.width = 5;
-            cd.snr = mode == 2 ? (m - lm) / lr : m / lr;
-            cands[slot] = cd;
+    const float *r = x + (int64_t)d * stride + l0;
+    float v[8];
+    if (l0 + 8u <= n && ((uintptr_t)r % 16) == 0) {
+        const float4 a = ((const float4 *)r)[0], b = ((const float4 *)r)[1];
+        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    } else {
+#pragma unroll
+        for (int i = 0; i < 8; ++i) v[i] = l0 + i < n ? r[i] : 0.0f;
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const uint32_t l = l0 + q;
+        if (l >= n - 4) break;
+        float lm = rm, lr = rr;
+        if (mode == 2) {
+            lm = cmean[l / spc];
+            lr = crms[l / spc];
+        }
+        const float m = med5(v[q], v[q + 1], v[q + 2], v[q + 3], v[q + 4]);
+        if (m - lm >= lr * thr) {
+            const uint32_t slot = atomicAdd(count, 1u);
+            if (slot < cap) {
+                mdsm_cand cd;
+                cd.dm = d;
+                cd.t = l;
+                cd.width = 5;
+                cd.snr = mode == 2 ? (m - lm) / lr : m / lr;
+                cands[slot] = cd;
+            }
         }
     }
 }
@@ -1368,7 +1385,7 @@ hipError_t launch_median5(const float *x, uint32_t ndms, uint32_t n, int64_t str
         if (mode == 0)
             hipLaunchKernelGGL(block_stats_finish, dim3(1), dim3(64), 0, s, s1, s2, ndms, n, stats);
     }
-    dim3 grid((n - 4 + 255) / 256, ndms);
+    dim3 grid(((n - 4 + 3) / 4 + 255) / 256, ndms);
     hipLaunchKernelGGL(median5_kernel, grid, dim3(256), 0, s, x, n, stride, dmv, (float)thr, mode,
                        cmean, crms, spc, cands, cap, count, stats, ndms);
     return hipGetLastError();

@@ -175,20 +175,25 @@ def _np_median5(x, dmv, thr):
     return res, lm, lr
 
 
-def test_median5_detector(gpu):
+@pytest.mark.parametrize("n,stride", [(4000, 4000), (4001, 4003), (4003, 4004)])
+def test_median5_detector(gpu, n, stride):
+    """Aligned rows (16-byte loads) and unaligned or ragged ones."""
     M = gpu
-    print("case median5", flush=True)
+    print(f"case median5 n={n} stride={stride}", flush=True)
     rng = np.random.default_rng(61)
-    D, n = 30, 4000
+    D = 30
     x = rng.integers(0, 100, (D, n)).astype(np.float32)
     x[4, 100:110] += 400
+    x[5, n - 7:n] += 400        # a burst in the last window
     x[0, 50:60] += 400          # DM 0 row: excluded by dm > 1
     dmv = np.arange(D, dtype=np.float32) * 0.5
-    dx = M.DeviceBuffer.from_host(x)
+    xs = np.zeros((D, stride), np.float32)
+    xs[:, :n] = x
+    dx = M.DeviceBuffer.from_host(xs)
     ddm = M.DeviceBuffer.from_host(dmv)
     cap = 4096
     dc, dn, dst = M.DeviceBuffer(cap * 16), M.DeviceBuffer(4), M.DeviceBuffer((2 + 2 * D) * 8)
-    M.detect_median5(dx, D, n, n, ddm, 3.0, dc, cap, dn, dst)
+    M.detect_median5(dx, D, n, stride, ddm, 3.0, dc, cap, dn, dst)
     M.synchronize()
     got = M.read_candidates(dc, dn, cap)
     exp, lm, lr = _np_median5(x, dmv, 3.0)
