@@ -653,6 +653,9 @@ hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s)
         grid = (unsigned)pgrid;
     } else {
         a.counter = nullptr;
+        // every item is resident at once: tile-major order lets the blocks of
+        // one tile, running side by side, share their input windows in L2
+        if (L.counter) a.group = 1;
     }
     hipLaunchKernelGGL(dedisperse_kernel, dim3(grid), dim3(kThreads), lds, s, a);
     return hipGetLastError();
