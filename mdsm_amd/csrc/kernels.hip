@@ -222,6 +222,33 @@ struct Layout {
     static constexpr int eb = (DT == DT_F32 && !kF32Two) ? 16 : 8;         // bytes per LDS entry
 };
 
+// Sample-by-sample fetch of 8 samples at a row end or an odd row start (zero
+// past in_len): out of line, it is rarely taken and would otherwise be
+// inlined into every staging loop.
+template <int DT>
+struct Raw8 {
+    uint32_t v[Layout<DT>::raw];
+};
+template <int DT>
+__device__ __attribute__((noinline)) Raw8<DT> fetch8_slow(const void *src, int64_t rowbase, int col,
+                                                          int in_len)
+{
+    Raw8<DT> r;
+#pragma unroll
+    for (int i = 0; i < Layout<DT>::raw; ++i) r.v[i] = 0u;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const uint32_t v = (col + j < in_len) ? load_u<DT>(src, rowbase + col + j) : 0u;
+        if constexpr (DT == DT_U8)
+            r.v[j >> 2] |= v << (8 * (j & 3));
+        else if constexpr (DT == DT_F32)
+            r.v[j] = v;
+        else
+            r.v[j >> 1] |= v << (16 * (j & 1));
+    }
+    return r;
+}
+
 // Global fetch of 8 consecutive samples (one LDS entry column) into raw
 // registers.
 template <int DT>
@@ -234,7 +261,11 @@ __device__ __forceinline__ void fetch8(const void *src, int64_t rowbase, int col
         const MDSM_GLOBAL_AS uint32_t *p = gptr<uint32_t>((const char *)src + (rowbase + col) * es);
 #pragma unroll
         for (int i = 0; i < n; ++i) raw[i] = p[i];
-    } else {
+    } else if constexpr (DT == DT_F16 || DT == DT_F32) {
+        const Raw8<DT> r = fetch8_slow<DT>(src, rowbase, col, in_len);
+#pragma unroll
+        for (int i = 0; i < n; ++i) raw[i] = r.v[i];
+    } else {  // integer data: inline (the out-of-line call costs this path more than it saves)
 #pragma unroll
         for (int i = 0; i < n; ++i) raw[i] = 0u;
 #pragma unroll
@@ -242,8 +273,6 @@ __device__ __forceinline__ void fetch8(const void *src, int64_t rowbase, int col
             const uint32_t v = (col + j < in_len) ? load_u<DT>(src, rowbase + col + j) : 0u;
             if constexpr (DT == DT_U8)
                 raw[j >> 2] |= v << (8 * (j & 3));
-            else if constexpr (DT == DT_F32)
-                raw[j] = v;
             else
                 raw[j >> 1] |= v << (16 * (j & 1));
         }
