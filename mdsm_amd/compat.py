@@ -116,3 +116,59 @@ def start_processing(data_read):
 
 def tearDownMDSM():
     _sig().tearDownMDSM()
+
+
+# ------------------------------------------------ host/device ABI (mdsm_wrapper.h)
+class DEVICE_INFO(C.Structure):
+    """src/mdsm/dedispersion_thread.h:9-20."""
+    _fields_ = [("multiprocessor_count", C.c_int), ("constant_memory", C.c_int),
+                ("shared_memory", C.c_int), ("register_count", C.c_int),
+                ("thread_count", C.c_int), ("clock_rate", C.c_int), ("device_id", C.c_int)]
+
+
+class DEVICES(C.Structure):
+    """src/mdsm/dedispersion_thread.h:22-29 (minTotalGlobalMem in KiB)."""
+    _fields_ = [("num_devices", C.c_int), ("minTotalGlobalMem", C.c_ulong),
+                ("devices", C.POINTER(DEVICE_INFO))]
+
+
+class THREAD_PARAMS(C.Structure):
+    """src/mdsm/dedispersion_thread.h:31-57; the lock and barriers are
+    pointers to glibc pthread_rwlock_t / pthread_barrier_t."""
+    _fields_ = [("iterations", C.c_int), ("maxiters", C.c_int), ("maxshift", C.c_int),
+                ("binsize", C.c_int), ("dmshifts", C.POINTER(C.c_float)),
+                ("inputsize", C.c_size_t), ("outputsize", C.c_size_t),
+                ("dm_output", C.c_size_t), ("dedispersed_size", C.c_size_t),
+                ("output", C.POINTER(C.c_float)), ("input", C.POINTER(C.c_float)),
+                ("device_id", C.c_int), ("thread_num", C.c_ushort),
+                ("num_threads", C.c_ushort), ("stop", C.c_ushort),
+                ("rw_lock", C.c_void_p), ("input_barrier", C.c_void_p),
+                ("output_barrier", C.c_void_p), ("start", C.c_long),
+                ("survey", C.POINTER(SURVEY))]
+
+
+def wrapper_lib():
+    """The library with call_initialise_devices / call_dedisperse typed."""
+    L = lib()
+    L.call_initialise_devices.restype = C.POINTER(DEVICES)
+    L.call_initialise_devices.argtypes = [C.POINTER(SURVEY)]
+    L.call_dedisperse.restype = C.c_void_p
+    L.call_dedisperse.argtypes = [C.c_void_p]
+    return L
+
+
+def call_initialise_devices(survey=None):
+    """List of DEVICE_INFO dicts and minTotalGlobalMem (KiB); ([], 0) when no
+    GPU qualifies.  The DEVICES block is freed here."""
+    p = wrapper_lib().call_initialise_devices(survey)
+    if not p:
+        return [], 0
+    d = p.contents
+    out = [{f: getattr(d.devices[i], f) for f, _ in DEVICE_INFO._fields_}
+           for i in range(d.num_devices)]
+    mem = d.minTotalGlobalMem
+    libc = C.CDLL(None)
+    libc.free.argtypes = [C.c_void_p]
+    libc.free(C.cast(d.devices, C.c_void_p))
+    libc.free(C.cast(p, C.c_void_p))
+    return out, mem

@@ -22,7 +22,7 @@ def test_abi_exports_every_declared_symbol():
             txt = open(os.path.join(ROOT, "include", h)).read()
             txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
             decl |= set(re.findall(r"\b(mdsm_\w+|initialiseMDSM|processSurveyParameters|"
-                                   r"next_chunk|start_processing|tearDownMDSM)\s*\(", txt))
+                                   r"next_chunk|start_processing|tearDownMDSM|call_\w+)\s*\(", txt))
     assert len(decl) > 20
     out = subprocess.run(["nm", "-D", "--defined-only", M.LIB_PATH], capture_output=True,
                          text=True, check=True).stdout
