@@ -172,3 +172,69 @@ def call_initialise_devices(survey=None):
     libc.free(C.cast(d.devices, C.c_void_p))
     libc.free(C.cast(p, C.c_void_p))
     return out, mem
+
+
+# ------------------------------------------------ MdsmModule adapter (mdsm_module.h)
+class MODULE_OUTPUT(C.Structure):
+    _fields_ = [("nseries", C.c_uint32), ("dm_values", C.POINTER(C.c_float)),
+                ("nsamp", C.POINTER(C.c_uint32)),
+                ("data", C.POINTER(C.POINTER(C.c_float)))]
+
+
+def _module_sig():
+    L = lib()
+    L.mdsm_module_create.restype = C.c_int
+    L.mdsm_module_create.argtypes = [C.c_char_p, C.c_int, C.c_int, C.POINTER(C.c_void_p)]
+    L.mdsm_module_run.restype = C.c_int
+    L.mdsm_module_run.argtypes = [C.c_void_p, C.c_void_p, C.c_uint32, C.c_uint32, C.c_uint32,
+                                  C.c_int64, C.c_int64, C.c_int, C.c_float, C.c_float,
+                                  C.c_double, C.c_double, C.POINTER(MODULE_OUTPUT)]
+    L.mdsm_module_survey.restype = C.POINTER(SURVEY)
+    L.mdsm_module_survey.argtypes = [C.c_void_p]
+    L.mdsm_module_destroy.restype = None
+    L.mdsm_module_destroy.argtypes = [C.c_void_p]
+    return L
+
+
+class MdsmModule:
+    """MdsmModule (src/pelican-mdsm/modules/MdsmModule.h:16-45) over the C ABI:
+    run() takes one blob of spectra [ntime][nsubbands][nchannels] float32 (or
+    None / ntime 0 for the end of the stream) and returns the dedispersed
+    series of a completed block as a list of (dm, samples) -- copies -- or []."""
+
+    def __init__(self, observation_file, create_output_blob=True, invert_channels=True):
+        self._L = _module_sig()
+        h = C.c_void_p()
+        rc = self._L.mdsm_module_create(str(observation_file).encode(), int(create_output_blob),
+                                        int(invert_channels), C.byref(h))
+        if rc:
+            raise RuntimeError(f"mdsm_module_create: {_err()}")
+        self._h = h
+
+    @property
+    def survey(self):
+        return self._L.mdsm_module_survey(self._h).contents
+
+    def run(self, spectra=None, weighted=False, mean=0.0, rms=0.0, timestamp=0.0,
+            block_rate=0.0):
+        out = MODULE_OUTPUT()
+        if spectra is None or spectra.shape[0] == 0:
+            rc = self._L.mdsm_module_run(self._h, None, 0, 0, 0, 0, 0, int(weighted), mean, rms,
+                                         timestamp, block_rate, C.byref(out))
+        else:
+            x = np.asarray(spectra, np.float32)
+            assert x.ndim == 3 and x.strides[2] == 4
+            rc = self._L.mdsm_module_run(self._h, x.ctypes.data, x.shape[0], x.shape[1],
+                                         x.shape[2], x.strides[0] // 4, x.strides[1] // 4,
+                                         int(weighted), mean, rms, timestamp, block_rate,
+                                         C.byref(out))
+        if rc:
+            raise RuntimeError(f"mdsm_module_run: {_err()}")
+        return [(out.dm_values[i],
+                 np.ctypeslib.as_array(out.data[i], (out.nsamp[i],)).copy())
+                for i in range(out.nseries)]
+
+    def close(self):
+        if self._h:
+            self._L.mdsm_module_destroy(self._h)
+            self._h = None

@@ -365,6 +365,7 @@ struct Slot {
     unsigned nsamp = 0;
     double timestamp = 0, rate = 0;
     float *mean = nullptr, *rms = nullptr;
+    unsigned spc = 0;                // samples per chunk of mean/rms, 0 = not used
     bool valid = false;
 };
 
@@ -393,6 +394,9 @@ struct Compat {
     // candidate file
     FILE *fp = nullptr;
     long written_samples = 0;
+    // new samples arrive in d_in, written on `stream` by the caller (the
+    // module adapter's GPU corner turn), not in the host input buffer
+    bool dev_input = false;
 };
 
 Compat *g = nullptr;
@@ -461,8 +465,8 @@ void write_candidates(Compat *c, Slot &sl)
         const double step = sl.rate * (double)ri.downsample;
         for (const mdsm_cand &cd : v) {
             const float dm = c->dmv[ri.first_dm + cd.dm];
-            if (sl.mean && s->useBruteForce) {
-                const unsigned ch = cd.t / std::max(1u, s->samplesPerChunk);
+            if (sl.spc) {
+                const unsigned ch = cd.t / sl.spc;
                 std::fprintf(c->fp, "%lf, %f, %f, %f, %f  \n", sl.timestamp + cd.t * step, dm,
                              cd.snr, sl.mean[ch], sl.rms[ch]);
             } else {
@@ -658,15 +662,18 @@ extern "C" int start_processing(unsigned int data_read)
     hipStream_t st = c->stream;
     bool ok = true;
     if (k == 0) {
-        ok = hipok(hipMemcpyAsync(c->d_in, c->in_host, S * C * 4, hipMemcpyHostToDevice, st), "H2D");
+        if (!c->dev_input)
+            ok = hipok(hipMemcpyAsync(c->d_in, c->in_host, S * C * 4, hipMemcpyHostToDevice, st),
+                       "H2D");
     } else {
         // overlap: last M samples of the previous block, kept on the device
         if (M)
             ok = hipok(hipMemcpy2DAsync(c->d_in, S * 4, c->d_tail, M * 4, M * 4, C,
                                         hipMemcpyDeviceToDevice, st), "overlap restore");
-        ok = ok && hipok(hipMemcpy2DAsync(c->d_in + M, S * 4, c->in_host + M, S * 4,
-                                          (size_t)c->nsamp * 4, C, hipMemcpyHostToDevice, st),
-                         "H2D");
+        if (!c->dev_input)
+            ok = ok && hipok(hipMemcpy2DAsync(c->d_in + M, S * 4, c->in_host + M, S * 4,
+                                              (size_t)c->nsamp * 4, C, hipMemcpyHostToDevice, st),
+                             "H2D");
     }
     // a short last block: zero the samples past the end
     if (ok && nb < c->nsamp)
@@ -676,8 +683,8 @@ extern "C" int start_processing(unsigned int data_read)
     if (ok && M)
         ok = hipok(hipMemcpy2DAsync(c->d_tail, M * 4, c->d_in + c->nsamp, S * 4, M * 4, C,
                                     hipMemcpyDeviceToDevice, st), "overlap save");
-    // the caller may rewrite the input buffer once the H2D has completed
-    ok = ok && hipok(hipStreamSynchronize(st), "H2D sync");
+    // the caller may rewrite the host input buffer once the H2D has completed
+    if (!c->dev_input) ok = ok && hipok(hipStreamSynchronize(st), "H2D sync");
     if (!ok) return 0;
     if (mdsm_dedisperse(c->plan, c->d_in, (int64_t)S, c->d_out, c->d_work, st)) return 0;
     // detection and D2H per range, in the reference's output layout
@@ -725,6 +732,7 @@ extern "C" int start_processing(unsigned int data_read)
     sl.rate = c->pend_rate;
     sl.mean = c->pend_mean;
     sl.rms = c->pend_rms;
+    sl.spc = (c->pend_mean && c->pend_rms && s->useBruteForce) ? s->samplesPerChunk : 0;
     sl.valid = true;
     c->pend_nsamp = 0;
     c->started++;
@@ -738,3 +746,13 @@ extern "C" void tearDownMDSM(void)
 }
 
 extern "C" const float *mdsm_compat_device_output(void) { return g ? g->d_out : nullptr; }
+
+int mdsm::compat_device_input(float **d_in, int64_t *stride, hipStream_t *stream)
+{
+    if (!g) return fail(MDSM_ERR_STATE, "no manager instance (initialiseMDSM)");
+    g->dev_input = true;
+    *d_in = g->d_in;
+    *stride = (int64_t)g->nsamp + g->M;
+    *stream = g->stream;
+    return MDSM_OK;
+}

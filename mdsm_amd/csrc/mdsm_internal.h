@@ -193,6 +193,13 @@ namespace mdsm {
 
 int upload_plan(mdsm_plan *p);
 
+// Switch the manager instance (initialiseMDSM) to device input: from now on
+// start_processing takes each block's new samples from d_in (rows of
+// `stride` floats; block 0 at column 0, later blocks at column maxshift),
+// written by the caller on `stream` before the call, instead of the host
+// input buffer.
+int compat_device_input(float **d_in, int64_t *stride, hipStream_t *stream);
+
 // Launch-shape defaults and the autotuner's search (plan.cpp): `apply` sets a
 // candidate, `step` enqueues one timed run on stream s.
 mdsm_tuning default_tuning();
