@@ -322,11 +322,14 @@ __device__ __forceinline__ void store_phases(char *row, int K, int k, const uint
 }
 
 // Register prefetch of one chunk: wave w owns rows w, w+kWaves, ... of the
-// chunk (<= 8 rows), each up to 4 entry columns per lane (K <= 256).
-constexpr int kPfRows = (8 + kWaves - 1) / kWaves, kPfIters = 4;
+// chunk (<= 8 rows), each up to pf_iters entry columns per lane: K <= 256
+// (W <= 1024), or K <= 384 (W <= 1536) for float16.
+constexpr int kPfRows = (8 + kWaves - 1) / kWaves;
+template <int DT>
+constexpr int pf_iters() { return max_window(DT) / 256; }
 
 template <int DT>
-using PreRegs = uint32_t[kPfRows][kPfIters][Layout<DT>::raw];
+using PreRegs = uint32_t[kPfRows][pf_iters<DT>()][Layout<DT>::raw];
 
 template <int DT>
 __device__ __forceinline__ void fetch_chunk(PreRegs<DT> &pre, const void *src, int64_t in_off,
@@ -342,7 +345,7 @@ __device__ __forceinline__ void fetch_chunk(PreRegs<DT> &pre, const void *src, i
         const int col0 = ab[i] + t0;  // a vector load: a scalar one would tie the prefetch to LGKM waits
         const bool aligned = ((rowbase * es) & 3) == 0;
 #pragma unroll
-        for (int it = 0; it < kPfIters; ++it) {
+        for (int it = 0; it < pf_iters<DT>(); ++it) {
             const int k = lane + 64 * it;
             if (64 * it >= K) break;
             if (k < K) fetch8<DT>(src, rowbase, col0 + 4 * k, in_len, aligned, pre[rr][it]);
@@ -360,7 +363,7 @@ __device__ __forceinline__ void store_chunk(const PreRegs<DT> &pre, char *buf, i
         if (i >= cc) break;
         char *row = buf + (size_t)i * rb;
 #pragma unroll
-        for (int it = 0; it < kPfIters; ++it) {
+        for (int it = 0; it < pf_iters<DT>(); ++it) {
             const int k = lane + 64 * it;
             if (64 * it >= K) break;
             if (k < K) store_phases<DT>(row, K, k, pre[rr][it]);
@@ -430,26 +433,39 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
             ao[r] += x.y;
         }
     };
-    // sum rows [0, cc) of the staged chunk into the accumulators
+    // sum rows [0, cc) of the staged chunk into the accumulators.  The next
+    // row's (pair's) offsets are loaded while this row's LDS reads are in
+    // flight, so the scalar-load latency hides under the LDS latency instead
+    // of adding to it; the scheduling barriers keep that order.  Offsets past
+    // the chunk read the next chunk's rows or the zero rows after the block's
+    // table, never beyond it.
     auto sum_chunk = [&](const char *lane_base, const int32_t *grow, int cc) {
         if constexpr (DT == DT_F32) {
-            for (int i = 0; i < cc; ++i) {
-                int oi[R];
-                const cint32_t *ri = as_const(grow + i * kDB);
+            int oi[R];
+            {
+                const cint32_t *r0 = as_const(grow);
 #pragma unroll
-                for (int r = 0; r < R; ++r) oi[r] = ri[r];
-                const char *rowi = lane_base;  // the table entry holds the row's place
+                for (int r = 0; r < R; ++r) oi[r] = r0[r];
+            }
+            for (int i = 0; i < cc; ++i) {
                 float4 x[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     if constexpr (kF32Two) {
-                        const float2 *q = (const float2 *)(rowi + oi[r]);
+                        const float2 *q = (const float2 *)(lane_base + oi[r]);
                         const float2 lo = q[0], hi = q[64];
                         x[r] = make_float4(lo.x, lo.y, hi.x, hi.y);
                     } else {
-                        x[r] = *(const float4 *)(rowi + oi[r]);
+                        x[r] = *(const float4 *)(lane_base + oi[r]);
                     }
                 }
+                __builtin_amdgcn_sched_barrier(0);
+                {
+                    const cint32_t *rn = as_const(grow + (i + 1) * kDB);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) oi[r] = rn[r];
+                }
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     f[r][0] += x[r].x;
@@ -459,22 +475,33 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 }
             }
         } else {
-            int i = 0;
-            for (; i + 1 < cc; i += 2) {
-                int oi[R], oj[R];
-                const cint32_t *ri = as_const(grow + i * kDB);
+            int oi[R], oj[R];
+            {
+                const cint32_t *r0 = as_const(grow);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    oi[r] = ri[r];
-                    oj[r] = ri[kDB + r];
+                    oi[r] = r0[r];
+                    oj[r] = r0[kDB + r];
                 }
-                const char *rowi = lane_base, *rowj = lane_base;
+            }
+            int i = 0;
+            for (; i + 1 < cc; i += 2) {
                 uint2 x[R], y[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    x[r] = *(const uint2 *)(rowi + oi[r]);
-                    y[r] = *(const uint2 *)(rowj + oj[r]);
+                    x[r] = *(const uint2 *)(lane_base + oi[r]);
+                    y[r] = *(const uint2 *)(lane_base + oj[r]);
                 }
+                __builtin_amdgcn_sched_barrier(0);
+                {
+                    const cint32_t *rn = as_const(grow + (i + 2) * kDB);
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        oi[r] = rn[r];
+                        oj[r] = rn[kDB + r];
+                    }
+                }
+                __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     if constexpr (kInt) {
@@ -487,13 +514,8 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 }
             }
             if (i < cc) {
-                int oi[R];
-                const cint32_t *ri = as_const(grow + i * kDB);
 #pragma unroll
-                for (int r = 0; r < R; ++r) oi[r] = ri[r];
-                const char *rowi = lane_base;
-#pragma unroll
-                for (int r = 0; r < R; ++r) add_entry(r, *(const uint2 *)(rowi + oi[r]));
+                for (int r = 0; r < R; ++r) add_entry(r, *(const uint2 *)(lane_base + oi[r]));
             }
         }
     };

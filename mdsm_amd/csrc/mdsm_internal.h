@@ -29,7 +29,6 @@ constexpr int kDB = kWaves * kRPW;   // DM trials per block
 constexpr int kGroups = MDSM_GROUPS; // integer path: 4-sample groups per lane, 256 apart
 constexpr int kPPL = 4 * kGroups;    // output samples per lane
 constexpr int kTile = 256 * kGroups; // output samples per tile
-constexpr int kMaxW = 1024;          // widest LDS row (floats)
 
 // Buffers a block can read or write.
 enum BufId : int32_t { BUF_IN = 0, BUF_OUT = 1, BUF_WORK = 2 };
@@ -88,6 +87,11 @@ inline int32_t lds_offset(int dtype, int W, int32_t o)
     return (o & 3) * K * 16 + (o >> 2) * 16;
 }
 inline bool int_path(int dtype) { return dtype == DT_U8 || dtype == DT_U16; }
+// Widest staged row (samples) of a block: bounded by the kernel's register
+// prefetch, 256 entry columns (1024 samples), or 384 (1536 samples) for
+// float16, whose wide subband windows would otherwise split blocks (float32
+// rows carry twice the prefetch registers and stay at 1024).
+constexpr int max_window(int dtype) { return dtype == DT_F16 ? 1536 : 1024; }
 constexpr int kLdsBudget = 79872;  // default LDS bytes per workgroup (two per CU: 2 x 78 KiB + static)
 constexpr int kLdsMax = 163840 - 1024;  // gfx950: 160 KiB per CU, less the kernel's static LDS
 constexpr int kChunkRows = 8;      // default channels per LDS chunk (2..8)

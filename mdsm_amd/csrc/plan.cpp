@@ -82,7 +82,7 @@ int validate_obs(const mdsm_obs *o)
 }
 
 // Appends one segment: trials [0, ntrials) over rows [0, nrows), split into
-// blocks of at most kDB trials whose LDS window fits kMaxW.
+// blocks of at most kDB trials whose LDS window fits max_window(dtype).
 struct SegSpec {
     int ntrials, nrows;
     std::function<int32_t(int, int)> sh;  // (trial, row) -> shift >= 0
@@ -118,7 +118,7 @@ int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
                 base[i] = lo & ~15;
                 relmax = std::max<int64_t>(relmax, hi - base[i]);
             }
-            if (round_up(kTile + relmax, 64) <= kMaxW || nt == 1) break;
+            if (round_up(kTile + relmax, 64) <= max_window(s.dtype) || nt == 1) break;
             nt = (nt + 1) / 2;
         }
         BlockDesc b{};
