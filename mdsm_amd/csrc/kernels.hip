@@ -436,17 +436,23 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
     // sum rows [0, cc) of the staged chunk into the accumulators.  The next
     // row's (pair's) offsets are loaded while this row's LDS reads are in
     // flight, so the scalar-load latency hides under the LDS latency instead
-    // of adding to it; the scheduling barriers keep that order.  Offsets past
-    // the chunk read the next chunk's rows or the zero rows after the block's
-    // table, never beyond it.
+    // of adding to it (the last row's load fetches the next chunk's first
+    // rows); the scheduling barriers keep that order.  Offsets past the block
+    // read the next block's rows or the zero rows after the segment's tables,
+    // never beyond them.
+    // oi/oj hold the offsets of the chunk's first row (pair) on entry and of
+    // the next chunk's on exit.
+    int oi[R], oj[R];
+    auto load_offsets = [&](const int32_t *g) {
+        const cint32_t *r0 = as_const(g);
+#pragma unroll
+        for (int r = 0; r < R; ++r) {
+            oi[r] = r0[r];
+            if constexpr (DT != DT_F32) oj[r] = r0[kDB + r];
+        }
+    };
     auto sum_chunk = [&](const char *lane_base, const int32_t *grow, int cc) {
         if constexpr (DT == DT_F32) {
-            int oi[R];
-            {
-                const cint32_t *r0 = as_const(grow);
-#pragma unroll
-                for (int r = 0; r < R; ++r) oi[r] = r0[r];
-            }
             for (int i = 0; i < cc; ++i) {
                 float4 x[R];
 #pragma unroll
@@ -475,15 +481,6 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 }
             }
         } else {
-            int oi[R], oj[R];
-            {
-                const cint32_t *r0 = as_const(grow);
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    oi[r] = r0[r];
-                    oj[r] = r0[kDB + r];
-                }
-            }
             int i = 0;
             for (; i + 1 < cc; i += 2) {
                 uint2 x[R], y[R];
@@ -514,14 +511,21 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 }
             }
             if (i < cc) {
+                uint2 x[R];
 #pragma unroll
-                for (int r = 0; r < R; ++r) add_entry(r, *(const uint2 *)(lane_base + oi[r]));
+                for (int r = 0; r < R; ++r) x[r] = *(const uint2 *)(lane_base + oi[r]);
+                __builtin_amdgcn_sched_barrier(0);
+                load_offsets(grow + cc * kDB);
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int r = 0; r < R; ++r) add_entry(r, x[r]);
             }
         }
     };
 
     int acc_rows = 0;
     PreRegs<DT> pre;
+    load_offsets(relb + wave * kRPW);
     __syncthreads();  // the previous item (persistent mode) is done with the buffers
     {
         const int cc0 = min(ccr, bd.nrows);
