@@ -489,6 +489,25 @@ def cpu_baseline_all_cores(plan, x, w, gpu_out, ndms, nout_sample=1 << 17):
             "seconds": secs, "matches_gpu": bool(np.array_equal(out_tm.T, gpu_out[:ndms, :nout]))}
 
 
+def error_vs_fp64(plan, x, dout, rows=8, nsamp=1 << 15):
+    """Max-abs and normwise relative error (max|gpu - fp64| / max|fp64|) of
+    the first `rows` DM rows of every range over their first `nsamp` output
+    samples, against the FP64 oracle on the same input."""
+    import oracle
+    worst_abs, worst_rel, per = 0.0, 0.0, []
+    for r, R in enumerate(plan.ranges):
+        nr, n = min(rows, R.ndms), min(nsamp, R.nout)
+        sh = plan.shift_table(r)[:nr]
+        need = min(x.shape[1], (n + int(sh.max()) + 1) * R.downsample)
+        ref = oracle.range_f64(np.ascontiguousarray(x[:, :need]), R.downsample, sh, n)
+        got = dout.download_rows(np.float32, nr, n, R.nout, offset=R.out_offset * 4)
+        ab = float(np.max(np.abs(got.astype(np.float64) - ref)))
+        rel = ab / max(float(np.max(np.abs(ref))), 1e-300)
+        worst_abs, worst_rel = max(worst_abs, ab), max(worst_rel, rel)
+        per.append({"range": r, "rows": nr, "samples": n, "max_abs": ab, "rel": rel})
+    return {"max_abs": worst_abs, "rel": worst_rel, "tolerance_rel": 1e-5, "ranges": per}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -612,6 +631,8 @@ def main():
                                               plan.ranges[0].nout))
         out["cpu_baseline"] = cpu_baseline(plan, x, w, gpu_rows, budget_dms=min(args.cpu_dms, ndm0))
         out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(plan, x, w, gpu_rows, ndm_mt)
+        # the CPU leg also checks sampled rows of every range against FP64
+        out["error_vs_fp64"] = error_vs_fp64(plan, x, dout)
     print(json.dumps(out), flush=True)
 
 

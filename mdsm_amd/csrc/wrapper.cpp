@@ -27,7 +27,12 @@ extern "C" DEVICES *call_initialise_devices(SURVEY *survey)
         return nullptr;
     }
     auto *d = (DEVICES *)std::malloc(sizeof(DEVICES));
-    d->devices = (DEVICE_INFO *)std::malloc((size_t)n * sizeof(DEVICE_INFO));
+    if (d) d->devices = (DEVICE_INFO *)std::malloc((size_t)n * sizeof(DEVICE_INFO));
+    if (!d || !d->devices) {
+        std::free(d);
+        fail(MDSM_ERR_NOMEM, "host allocation failed");
+        return nullptr;
+    }
     d->num_devices = 0;
     d->minTotalGlobalMem = 1024ul * 1024ul * 16ul;  // KiB, as the reference starts it
     for (int i = 0; i < n; ++i) {
