@@ -113,7 +113,10 @@ int enqueue(mdsm_batch *b, const void *const *ins, const int64_t *strides, float
         uint32_t grid = 0, group = 1;
         for (size_t i = 0; i < n; ++i) {
             const mdsm_plan *p = b->plans[i];
-            if (k < p->stages.size()) L.lds_bytes = std::max(L.lds_bytes, p->stages[k].lds_bytes);
+            if (k < p->stages.size()) {
+                L.lds_bytes = std::max(L.lds_bytes, p->stages[k].lds_bytes);
+                L.dtypes |= p->stages[k].dtypes;
+            }
             persistent = persistent || p->tune.persistent;
             grid = std::max(grid, p->tune.grid);
             group = std::max(group, p->tune.tile_group);

@@ -380,6 +380,35 @@ __device__ __forceinline__ float half_hi(uint32_t v)
     return (float)__builtin_bit_cast(_Float16, (uint16_t)(v >> 16));
 }
 
+// Lane k of each row of 16 lanes, to every lane of that row (DPP
+// row_newbcast; folded into the consuming VALU instruction by the compiler).
+template <int K>
+__device__ __forceinline__ int bcast16_c(int v)
+{
+    return __builtin_amdgcn_update_dpp(0, v, 0x150 + K, 0xf, 0xf, false);
+}
+__device__ __forceinline__ int bcast16(int v, int k)
+{
+    switch (k) {
+    case 0: return bcast16_c<0>(v);
+    case 1: return bcast16_c<1>(v);
+    case 2: return bcast16_c<2>(v);
+    case 3: return bcast16_c<3>(v);
+    case 4: return bcast16_c<4>(v);
+    case 5: return bcast16_c<5>(v);
+    case 6: return bcast16_c<6>(v);
+    case 7: return bcast16_c<7>(v);
+    case 8: return bcast16_c<8>(v);
+    case 9: return bcast16_c<9>(v);
+    case 10: return bcast16_c<10>(v);
+    case 11: return bcast16_c<11>(v);
+    case 12: return bcast16_c<12>(v);
+    case 13: return bcast16_c<13>(v);
+    case 14: return bcast16_c<14>(v);
+    default: return bcast16_c<15>(v);
+    }
+}
+
 // Chunks of ccr <= 8 channels are double-buffered in LDS; the next chunk's
 // global loads are in flight (in registers) while the current chunk is
 // summed.  Each wave reads its kRPW trials' offsets of a channel (pair) with
@@ -443,7 +472,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
     // oi/oj hold the offsets of the chunk's first row (pair) on entry and of
     // the next chunk's on exit.
     int oi[R], oj[R];
-    auto load_offsets = [&](const int32_t *g) {
+    auto load_soffsets = [&](const int32_t *g) {
         const cint32_t *r0 = as_const(g);
 #pragma unroll
         for (int r = 0; r < R; ++r) {
@@ -451,7 +480,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
             if constexpr (DT != DT_F32) oj[r] = r0[kDB + r];
         }
     };
-    auto sum_chunk = [&](const char *lane_base, const int32_t *grow, int cc) {
+    auto sum_chunk_s = [&](const char *lane_base, const int32_t *grow, int cc) {
         if constexpr (DT == DT_F32) {
             for (int i = 0; i < cc; ++i) {
                 float4 x[R];
@@ -515,7 +544,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
 #pragma unroll
                 for (int r = 0; r < R; ++r) x[r] = *(const uint2 *)(lane_base + oi[r]);
                 __builtin_amdgcn_sched_barrier(0);
-                load_offsets(grow + cc * kDB);
+                load_soffsets(grow + cc * kDB);
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int r = 0; r < R; ++r) add_entry(r, x[r]);
@@ -523,9 +552,100 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         }
     };
 
+
+    // Offsets: a wave's trials of a channel pair sit in one VGPR, lane 16q + m
+    // holding (row 2p + m/8, trial m%8) for every q, and a row_newbcast DPP
+    // folded into the address add hands lane m's value to every lane (no
+    // extra instruction).  A chunk's offset registers (up to 4 pairs) are
+    // vector loads issued when the previous chunk's LDS reads are done, in
+    // flight across its staging stores and the barrier; they read rows up to
+    // c0 + 7: the block's table, the next block's or the zero rows after the
+    // segment's tables.
+    int vo[4];
+    const int orow = (lane >> 3) & 1, ocol = lane & 7;
+    auto load_offsets = [&](int (&v)[4], int c0) {
+        const MDSM_GLOBAL_AS int32_t *g =
+            gptr<int32_t>(relb + (size_t)(c0 + orow) * kDB + wave * kRPW + ocol);
+#pragma unroll
+        for (int p = 0; p < 4; ++p) v[p] = g[2 * p * kDB];
+    };
+    // sum rows [0, cc) of the staged chunk into the accumulators
+    auto sum_chunk = [&](const char *lane_base, int cc) {
+        if constexpr (DT == DT_F32) {
+            auto row = [&](int o_pair, int half) {
+                float4 x[R];
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int o = bcast16(o_pair, 8 * half + r);
+                    if constexpr (kF32Two) {
+                        const float2 *q = (const float2 *)(lane_base + o);
+                        const float2 lo = q[0], hi = q[64];
+                        x[r] = make_float4(lo.x, lo.y, hi.x, hi.y);
+                    } else {
+                        x[r] = *(const float4 *)(lane_base + o);
+                    }
+                }
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    f[r][0] += x[r].x;
+                    f[r][1] += x[r].y;
+                    f[r][2] += x[r].z;
+                    f[r][3] += x[r].w;
+                }
+            };
+#pragma unroll
+            for (int pr = 0; pr < 4; ++pr) {
+                if (2 * pr >= cc) break;
+                __builtin_amdgcn_sched_barrier(0);
+                row(vo[pr], 0);
+                if (2 * pr + 1 >= cc) break;
+                __builtin_amdgcn_sched_barrier(0);
+                row(vo[pr], 1);
+            }
+        } else {
+#pragma unroll
+            for (int pr = 0; pr < 4; ++pr) {
+                const int i = 2 * pr;
+                if (i >= cc) break;
+                // one pair's reads at a time: hoisting the next pair's reads
+                // above these adds would need twice the data registers
+                __builtin_amdgcn_sched_barrier(0);
+                if (i + 1 < cc) {
+                    uint2 x[R], y[R];
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        x[r] = *(const uint2 *)(lane_base + bcast16(vo[pr], r));
+                        y[r] = *(const uint2 *)(lane_base + bcast16(vo[pr], 8 + r));
+                    }
+#pragma unroll
+                    for (int r = 0; r < R; ++r) {
+                        if constexpr (kInt) {
+                            ae[r] = ae[r] + x[r].x + y[r].x;
+                            ao[r] = ao[r] + x[r].y + y[r].y;
+                        } else {
+                            add_entry(r, x[r]);
+                            add_entry(r, y[r]);
+                        }
+                    }
+                } else {
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        add_entry(r, *(const uint2 *)(lane_base + bcast16(vo[pr], r)));
+                }
+            }
+        }
+    };
+
+    // uint8 rows take the per-lane offset registers; uint16 and the float
+    // paths, whose wider prefetch or float32 accumulators leave no VGPRs to
+    // spare at 4 waves per SIMD, the scalar offsets
+    constexpr bool kVOff = DT == DT_U8;
     int acc_rows = 0;
     PreRegs<DT> pre;
-    load_offsets(relb + wave * kRPW);
+    if constexpr (kVOff)
+        load_offsets(vo, 0);
+    else
+        load_soffsets(relb + wave * kRPW);
     __syncthreads();  // the previous item (persistent mode) is done with the buffers
     {
         const int cc0 = min(ccr, bd.nrows);
@@ -548,9 +668,17 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 acc_rows = 0;
             }
         }
-        sum_chunk(lds + (ch & 1) * bufsz + lane * kEB, relb + (size_t)c0 * kDB + wave * kRPW, cc);
+        if constexpr (kVOff)
+            sum_chunk(lds + (ch & 1) * bufsz + lane * kEB, cc);
+        else
+            sum_chunk_s(lds + (ch & 1) * bufsz + lane * kEB, relb + (size_t)c0 * kDB + wave * kRPW,
+                        cc);
         acc_rows += cc;
-        if (more) store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, rb, K, cc1, wave, lane);
+        if (more) {
+            // in flight across the staging stores and the barrier
+            if constexpr (kVOff) load_offsets(vo, c1);
+            store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, rb, K, cc1, wave, lane);
+        }
         __syncthreads();
     }
     flush();
@@ -593,6 +721,15 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
     }
 }
 
+// Kernel classes: the dtypes a dedispersion kernel is built for.  Each class
+// is its own kernel with its own register allocation, so the integer path's
+// offset registers do not crowd the float paths: launches of float blocks
+// only take the float kernel (config 3: 20.1 -> 19.6 ms), every other launch
+// the kernel of all dtypes (an integer-only kernel was 1.5% slower than it
+// on config 2).
+constexpr int kClsInt = 1, kClsFloat = 2, kClsAll = 3;
+
+template <int kCls>
 __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, int wave,
                                                 int lane, char *lds)
 {
@@ -623,25 +760,36 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
                 uniform_ptr(jb.out), uniform_ptr(jb.work), jb.in_stride};
     const int t0 = tile_i * kTile;
     // float blocks of at most kDB/2 trials run kRPW/2 trials per wave
-    if (bd.rpw == kRPW) {
-        switch (bd.dtype) {
-        case DT_U8: dedisperse_lds_t<DT_U8, kRPW>(ctx, bd, t0, wave, lane, lds, a.one); break;
-        case DT_U16: dedisperse_lds_t<DT_U16, kRPW>(ctx, bd, t0, wave, lane, lds, a.one); break;
-        case DT_F16: dedisperse_lds_t<DT_F16, kRPW>(ctx, bd, t0, wave, lane, lds, a.one); break;
-        default: dedisperse_lds_t<DT_F32, kRPW>(ctx, bd, t0, wave, lane, lds, a.one); break;
+    if constexpr ((kCls & kClsInt) != 0) {
+        if (bd.dtype == DT_U8) {
+            dedisperse_lds_t<DT_U8, kRPW>(ctx, bd, t0, wave, lane, lds, a.one);
+            return;
         }
-    } else {  // float data only (integer blocks always run kRPW trials per wave)
+        if (bd.dtype == DT_U16) {
+            dedisperse_lds_t<DT_U16, kRPW>(ctx, bd, t0, wave, lane, lds, a.one);
+            return;
+        }
+    }
+    if constexpr ((kCls & kClsFloat) != 0) {
         constexpr int H = kRPW / 2;
-        if (bd.dtype == DT_F16)
-            dedisperse_lds_t<DT_F16, H>(ctx, bd, t0, wave, lane, lds, a.one);
-        else
-            dedisperse_lds_t<DT_F32, H>(ctx, bd, t0, wave, lane, lds, a.one);
+        if (bd.rpw == kRPW) {
+            if (bd.dtype == DT_F16)
+                dedisperse_lds_t<DT_F16, kRPW>(ctx, bd, t0, wave, lane, lds, a.one);
+            else
+                dedisperse_lds_t<DT_F32, kRPW>(ctx, bd, t0, wave, lane, lds, a.one);
+        } else {  // integer blocks always run kRPW trials per wave
+            if (bd.dtype == DT_F16)
+                dedisperse_lds_t<DT_F16, H>(ctx, bd, t0, wave, lane, lds, a.one);
+            else
+                dedisperse_lds_t<DT_F32, H>(ctx, bd, t0, wave, lane, lds, a.one);
+        }
     }
 }
 
 #ifndef MDSM_DEDISP_WAVES
 #define MDSM_DEDISP_WAVES 4
 #endif
+template <int kCls>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MDSM_DEDISP_WAVES)))
 dedisperse_kernel(DedispArgs a)
 {
@@ -656,7 +804,7 @@ dedisperse_kernel(DedispArgs a)
         const int64_t q = a.items >> 3, rem = a.items & 7;
         const int64_t xcd = g & 7, loc = g >> 3;
         const int64_t f = xcd * q + (xcd < rem ? xcd : rem) + loc;
-        dedisperse_item(a, f, wave, lane, lds_raw);
+        dedisperse_item<kCls>(a, f, wave, lane, lds_raw);
         return;
     }
     // Persistent mode: workgroups pull items from a device counter until the
@@ -672,7 +820,7 @@ dedisperse_kernel(DedispArgs a)
         const uint32_t fhi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)fv >> 32));
         const int64_t f = (int64_t)(((uint64_t)fhi << 32) | flo);
         if (f >= a.items) break;
-        dedisperse_item(a, f, wave, lane, lds_raw);
+        dedisperse_item<kCls>(a, f, wave, lane, lds_raw);
     }
 }
 
@@ -691,12 +839,17 @@ hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s)
     for (int i = 0; i < L.njobs; ++i) a.jobs[i] = L.jobs[i];
     const size_t lds = (size_t)L.lds_bytes;
     if (lds > (size_t)kLdsMax) return hipErrorInvalidValue;
-    static bool lds_attr = false;  // allow the full gfx950 LDS to one workgroup, once
-    if (!lds_attr) {
-        hipError_t e = hipFuncSetAttribute((const void *)dedisperse_kernel,
+    const bool ints = (L.dtypes & ((1 << DT_U8) | (1 << DT_U16))) != 0;
+    const bool floats = (L.dtypes & ((1 << DT_F16) | (1 << DT_F32))) != 0;
+    const int cls = (floats && !ints) ? kClsFloat : kClsAll;
+    void (*kern)(DedispArgs) =
+        cls == kClsFloat ? dedisperse_kernel<kClsFloat> : dedisperse_kernel<kClsAll>;
+    static bool lds_attr[4] = {false, false, false, false};  // full gfx950 LDS to one workgroup, once
+    if (!lds_attr[cls]) {
+        hipError_t e = hipFuncSetAttribute((const void *)kern,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax);
         if (e != hipSuccess) return e;
-        lds_attr = true;
+        lds_attr[cls] = true;
     }
     unsigned grid = (unsigned)L.items;
     // A persistent grid at least as large as the queue would give every
@@ -712,7 +865,7 @@ hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s)
         // one tile, running side by side, share their input windows in L2
         if (L.counter) a.group = 1;
     }
-    hipLaunchKernelGGL(dedisperse_kernel, dim3(grid), dim3(kThreads), lds, s, a);
+    hipLaunchKernelGGL(kern, dim3(grid), dim3(kThreads), lds, s, a);
     return hipGetLastError();
 }
 
@@ -725,6 +878,7 @@ hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in
     L.nseg = st.seg_end - st.seg_begin;
     L.items = st.items;
     L.lds_bytes = st.lds_bytes;
+    L.dtypes = st.dtypes;
     L.group = (int32_t)p->tune.tile_group;
     if (p->tune.persistent) {
         L.counter = p->dev.counter + (&st - p->stages.data());

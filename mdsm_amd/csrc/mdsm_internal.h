@@ -126,6 +126,7 @@ struct Stage {
     int64_t items = 0;
     int32_t wmax = 0;
     int32_t lds_bytes = 0;   // dynamic LDS of the launch
+    int32_t dtypes = 0;      // bit d set when a block of DType d is in the launch
 };
 
 struct DecimStep {
@@ -165,6 +166,7 @@ struct LaunchSpec {
     uint32_t *counter = nullptr;    // persistent mode: device work counter (zeroed here)
     int32_t grid = 0;               // persistent mode: workgroups (0 = one per item)
     int32_t group = 1;              // tiles per item group
+    int32_t dtypes = 0;             // bit d: blocks of DType d in the launch (0 = any)
     int32_t njobs = 0;
     JobBufs jobs[kMaxJobs];
 };

@@ -160,6 +160,7 @@ int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
             }
         }
         st.wmax = std::max(st.wmax, b.W);
+        st.dtypes |= 1 << b.dtype;
         p->blocks.push_back(b);
         j0 += nt;
     }

@@ -309,6 +309,7 @@ int launch_batch(mdsm_server *s, Batch *bt)
                 jobof.push_back((int)ji);
             }
             L.lds_bytes = std::max(L.lds_bytes, sg.lds_bytes);
+            L.dtypes |= sg.dtypes;
         }
         std::vector<size_t> idx(order.size());
         for (size_t i = 0; i < idx.size(); ++i) idx[i] = i;
