@@ -725,7 +725,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
 // is its own kernel with its own register allocation, so the integer path's
 // offset registers do not crowd the float paths: launches of float blocks
 // only take the float kernel (config 3: 20.1 -> 19.6 ms), every other launch
-// the kernel of all dtypes (an integer-only kernel was 1.5% slower than it
+// the kernel of all dtypes (an integer-only kernel was about 1% slower than it
 // on config 2).
 constexpr int kClsInt = 1, kClsFloat = 2, kClsAll = 3;
 
