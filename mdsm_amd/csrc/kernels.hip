@@ -411,10 +411,10 @@ __device__ __forceinline__ int bcast16(int v, int k)
 
 // Chunks of ccr <= 8 channels are double-buffered in LDS; the next chunk's
 // global loads are in flight (in registers) while the current chunk is
-// summed.  Each wave reads its kRPW trials' offsets of a channel (pair) with
-// scalar loads and issues all of that channel (pair)'s LDS reads before the
-// adds.  `one` is 1.0f, passed at run time so that x*one+acc stays a fused
-// convert-and-add.
+// summed.  Each wave takes its kRPW trials' offsets of a channel (pair) from
+// per-lane offset registers (uint8) or scalar loads (the other dtypes) and
+// issues all of that channel (pair)'s LDS reads before the adds.  `one` is
+// 1.0f, passed at run time so that x*one+acc stays a fused convert-and-add.
 template <int DT, int R>
 __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
                                  int lane, char *lds, float one)
