@@ -1225,12 +1225,37 @@ __global__ void __launch_bounds__(256)
     __shared__ double p1[4], p2[4];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const float *row = x + (int64_t)blockIdx.x * stride;
-    double s1 = 0.0, s2 = 0.0;
-    for (uint32_t t = tid; t < n; t += 256) {
-        const double v = (double)row[t];
-        s1 += v;
-        s2 += v * v;
+    // 16-byte loads where the row allows, four of them in flight per thread,
+    // into four independent accumulator pairs (FP64 sums of float32 values:
+    // exact for the integer-valued rows of uint8 dedispersion in any order)
+    double a1[4] = {0.0, 0.0, 0.0, 0.0}, a2[4] = {0.0, 0.0, 0.0, 0.0};
+    uint32_t t = 0;
+    if (((uintptr_t)row & 15) == 0) {
+        const uint32_t n16 = n & ~1023u;  // whole rounds of 256 threads x 4 floats
+        for (; t < n16; t += 4096) {
+            float4 v[4];
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const uint32_t i = t + 1024u * j + 4u * tid;
+                v[j] = make_float4(0.f, 0.f, 0.f, 0.f);
+                if (i + 3 < n16) v[j] = *gptr<float4>(row + i);
+            }
+#pragma unroll
+            for (int j = 0; j < 4; ++j) {
+                const double e0 = v[j].x, e1 = v[j].y, e2 = v[j].z, e3 = v[j].w;
+                a1[j] += (e0 + e1) + (e2 + e3);
+                a2[j] += (e0 * e0 + e1 * e1) + (e2 * e2 + e3 * e3);
+            }
+        }
+        t = n16;
     }
+    for (uint32_t i = t + tid; i < n; i += 256) {
+        const double v = (double)row[i];
+        a1[0] += v;
+        a2[0] += v * v;
+    }
+    double s1 = (a1[0] + a1[1]) + (a1[2] + a1[3]);
+    double s2 = (a2[0] + a2[1]) + (a2[2] + a2[3]);
     s1 = wave_sum_f64(s1);
     s2 = wave_sum_f64(s2);
     if (lane == 0) {
