@@ -561,6 +561,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
     // flight across its staging stores and the barrier; they read rows up to
     // c0 + 7: the block's table, the next block's or the zero rows after the
     // segment's tables.
+    static_assert(kRPW == 8, "the offset registers hold 2 channels x 8 trials per 16 lanes");
     int vo[4];
     const int orow = (lane >> 3) & 1, ocol = lane & 7;
     auto load_offsets = [&](int (&v)[4], int c0) {
