@@ -246,3 +246,17 @@ def test_module_subband_series_and_labels(gpu, tmp_path):
                                       .astype(np.float64)), (b, d)
                 i += 1
         assert i == len(o)
+
+
+@pytest.mark.gpu
+def test_module_rejects_bad_blobs(gpu, tmp_path):
+    """Channel counts that do not match the observation, and blobs longer
+    than a block, are errors (the module stays usable)."""
+    C, nsamp = 64, 4096
+    mod = compat.MdsmModule(_write(tmp_path, _brute_xml(tmp_path, C, nsamp, 16, 1.0, 50.0)))
+    with pytest.raises(RuntimeError):
+        mod.run(np.zeros((100, 4, 8), np.float32))          # 32 channels, not 64
+    with pytest.raises(RuntimeError):
+        mod.run(np.zeros((nsamp + 1, 4, 16), np.float32))  # longer than a block
+    assert mod.run(np.zeros((100, 4, 16), np.float32)) == []
+    mod.close()

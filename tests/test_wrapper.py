@@ -181,3 +181,19 @@ def test_initialise_devices_lists_the_gpu(gpu):
     assert d["device_id"] == 0 and d["multiprocessor_count"] >= 1
     assert d["thread_count"] >= 256 and d["shared_memory"] >= 65536
     assert mem > 2.5 * 1024 * 1024
+
+
+@pytest.mark.gpu
+def test_worker_with_short_maxshift_writes_zeros(gpu):
+    """A manager whose maxshift is shorter than the trials' delays: the
+    worker reports it, keeps the protocol and writes zeros (the reference
+    would read past its input rows)."""
+    C_, nsamp = 64, 2048
+    s = H.brute_survey(C_, nsamp, FCH1, -300.0 / 64, TSAMP, 0.0, 2.0, 32, 1)
+    m = H.Manager(C.pointer(s), 1, 8, 32 * nsamp)
+    m.output[:] = 5.0
+    x = np.ones((C_, nsamp + 8), np.float32)
+    o1 = m.block(x)
+    o2 = m.block(np.ones((C_, nsamp), np.float32))
+    last = m.finish()
+    assert o1 is None and not o2.any() and not last.any()
