@@ -14,6 +14,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 
 #include "mdsm_internal.h"
 
@@ -845,12 +846,14 @@ hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s)
     const int cls = (floats && !ints) ? kClsFloat : kClsAll;
     void (*kern)(DedispArgs) =
         cls == kClsFloat ? dedisperse_kernel<kClsFloat> : dedisperse_kernel<kClsAll>;
-    static bool lds_attr[4] = {false, false, false, false};  // full gfx950 LDS to one workgroup, once
-    if (!lds_attr[cls]) {
+    // the full gfx950 LDS to one workgroup, once per kernel (host threads may
+    // launch concurrently; setting the attribute twice is harmless)
+    static std::atomic<bool> lds_attr[4];
+    if (!lds_attr[cls].load(std::memory_order_acquire)) {
         hipError_t e = hipFuncSetAttribute((const void *)kern,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax);
         if (e != hipSuccess) return e;
-        lds_attr[cls] = true;
+        lds_attr[cls].store(true, std::memory_order_release);
     }
     unsigned grid = (unsigned)L.items;
     // A persistent grid at least as large as the queue would give every
