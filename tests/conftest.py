@@ -25,7 +25,7 @@ def rel_err(got, ref64):
 
 @pytest.fixture(scope="session")
 def gpu():
-    """Skip-free GPU fixture: on a GPU box the library must load and see a device."""
+    """Skip-free GPU fixture: on a machine with a GPU the library must load and see a device."""
     import mdsm_amd as M
     n = M.device_count()
     if n < 1:

@@ -71,7 +71,7 @@ def test_plan_tables_and_layout_match_oracle(cfg, rule):
 
 
 def test_config_maxshifts_match_reference():
-    """maxshift of each BASELINE configuration under the reference rule."""
+    """maxshift of each benchmark configuration (configs 1-3) under the reference rule."""
     p1 = M.Plan.brute(256, 1500.0, -0.390625, 64e-6, 65536, "f32", [(0.0, 0.2, 512, 1)])
     assert p1.ranges[0].maxshift == 433
     p2 = M.Plan.brute(1024, 1500.0, -300.0 / 1024, 64e-6, 1 << 20, "u8",
@@ -175,6 +175,6 @@ def test_synth_pulse_on_ns_track():
 
 
 def test_library_reports_no_device_or_a_device():
-    # loading never fails; device_count is 0 here and >= 1 on a GPU box
+    # loading never fails; device_count is 0 without a GPU and >= 1 with one
     assert M.device_count() >= 0
     assert "gfx950" in M.version()
