@@ -168,6 +168,17 @@ def serve(M, w, args):
     staged_s = per_block(lambda: plan.dedisperse(din, N + M_over, dout))
     plain_rate = plan.info.dm_chan_samples / plain_s
     staged_rate = plan.info.dm_chan_samples / staged_s
+    cpu = {}
+    if not args.no_cpu:
+        # the reference CPU dedisperser on a bounded sample of one beam's block
+        # (the float32 copy the plain comparison runs on), checked against the
+        # staged float32 rows left in dout
+        ndm0 = plan.ranges[0].ndms
+        ndm_mt = min(ndm0, 4 * args.cpu_dms)
+        rows = dout.download(np.float32, (max(min(args.cpu_dms, ndm0), ndm_mt),
+                                          plan.ranges[0].nout))
+        cpu["cpu_baseline"] = cpu_baseline(plan, xin, w, rows, budget_dms=min(args.cpu_dms, ndm0))
+        cpu["cpu_baseline_all_cores"] = cpu_baseline_all_cores(plan, xin, w, rows, ndm_mt)
     return {
         "metric": "DM*chan*samples/s (served, clip+dedisperse+boxcar)",
         "value": thr_units, "unit": "DM*chan*samples/s", "n_gpus": 1,
@@ -186,6 +197,7 @@ def serve(M, w, args):
         "served_vs_plain": thr_units / plain_rate,
         "served_vs_staged_f32": thr_units / staged_rate,
         "server_stats": {k: int(v) for k, v in st.items()},
+        **cpu,
     }
 
 
