@@ -520,6 +520,25 @@ def error_vs_fp64(plan, x, dout, rows=8, nsamp=1 << 15):
     return {"max_abs": worst_abs, "rel": worst_rel, "tolerance_rel": 1e-5, "ranges": per}
 
 
+def subband_error_vs_fp64(plan, x, w, dout, nsamp=2048):
+    """Max-abs and normwise relative error of every DM row of a one-pass
+    subband plan over its first `nsamp` output samples, against the FP64
+    oracle of the two-stage rule on the same input window."""
+    import oracle
+    R = plan.ranges[0]
+    q = w["passes"][0]
+    n = min(nsamp, R.nout)
+    co = plan.shift_table(0)
+    fi = plan.fine_table(0, w["nsubs"])
+    win = np.ascontiguousarray(x[:, :R.maxshift + n])
+    ref = oracle.subband_f64(win, 1, w["nsubs"], q["ncalls"], q["calldms"], co, fi,
+                             n + int(fi.max()), n)
+    got = dout.download_rows(np.float32, R.ndms, n, R.nout)
+    ab = float(np.max(np.abs(got.astype(np.float64) - ref)))
+    return {"max_abs": ab, "rel": ab / max(float(np.max(np.abs(ref))), 1e-300),
+            "tolerance_rel": 1e-5, "rows": int(R.ndms), "samples": n}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -645,6 +664,9 @@ def main():
         out["cpu_baseline_all_cores"] = cpu_baseline_all_cores(plan, x, w, gpu_rows, ndm_mt)
         # the CPU leg also checks sampled rows of every range against FP64
         out["error_vs_fp64"] = error_vs_fp64(plan, x, dout)
+    elif not args.no_cpu and "passes" in w and len(w["passes"]) == 1:
+        print("[bench] FP64 check", file=sys.stderr, flush=True)
+        out["error_vs_fp64"] = subband_error_vs_fp64(plan, x, w, dout)
     print(json.dumps(out), flush=True)
 
 
