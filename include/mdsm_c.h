@@ -498,7 +498,15 @@ int mdsm_server_poll(mdsm_server *srv, mdsm_result *res, mdsm_cand *cands, uint3
 int mdsm_server_output(mdsm_server *srv, uint32_t beam_id, uint64_t seq, const float **d_out);
 int mdsm_server_release(mdsm_server *srv, uint32_t beam_id, uint64_t seq);
 
-/* Block until every pushed block has completed (results stay queued). */
+/* Pause (hold != 0) or resume admissions: while held, pushed blocks queue
+ * and nothing new is launched, so that a burst pushed in the meantime is
+ * admitted as one batch on resume (blocks already on the GPU run on).  No
+ * reference counterpart: the reference's manager has no batching.
+ * mdsm_server_drain resumes admissions. */
+int mdsm_server_hold(mdsm_server *srv, int hold);
+
+/* Block until every pushed block has completed (results stay queued).
+ * Resumes admissions if they were held. */
 int mdsm_server_drain(mdsm_server *srv);
 int mdsm_server_get_stats(mdsm_server *srv, mdsm_server_stats *st);
 /* Steady clock in seconds (the clock of t_arrival/t_admit/t_done). */

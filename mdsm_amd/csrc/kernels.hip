@@ -1133,9 +1133,10 @@ __global__ void __launch_bounds__(256)
                    int32_t *ncl_out)
 {
     __shared__ uint32_t hist[256];
+    __shared__ uint32_t whist[4][256];  // one histogram per wave: a quarter of the atomic contention
     __shared__ int sh_med, sh_mad;
     __shared__ uint32_t sh_cnt;
-    const int tid = threadIdx.x, lane = tid & 63;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     const int64_t c = blockIdx.x;
     const uint8_t *x = in + c * in_stride + t0;
     uint8_t *o = out + c * out_stride + t0;
@@ -1154,10 +1155,13 @@ __global__ void __launch_bounds__(256)
         }
         for (uint32_t t = 16u * nv + tid; t < n; t += 256) fn(t, (int)x[t]);
     };
-    hist[tid] = 0;
+#pragma unroll
+    for (int w = 0; w < 4; ++w) whist[w][tid] = 0;
     if (tid == 0) sh_cnt = 0;
     __syncthreads();
-    for_each([&](uint32_t, int v) { atomicAdd(&hist[v], 1u); });
+    for_each([&](uint32_t, int v) { atomicAdd(&whist[wv][v], 1u); });
+    __syncthreads();
+    hist[tid] = (whist[0][tid] + whist[1][tid]) + (whist[2][tid] + whist[3][tid]);
     __syncthreads();
     if (tid < 64) {
         const int m = hist_rank(hist, rank, lane);
@@ -1165,12 +1169,17 @@ __global__ void __launch_bounds__(256)
     }
     __syncthreads();
     const int med = sh_med;
-    hist[tid] = 0;
-    __syncthreads();
-    for_each([&](uint32_t, int v) { atomicAdd(&hist[abs(v - med)], 1u); });
+    // the histogram of |x - med| follows from the value histogram: no second
+    // pass over the row
+    {
+        const int d = tid;
+        const uint32_t up = med + d <= 255 ? hist[med + d] : 0u;
+        const uint32_t dn = (d > 0 && med - d >= 0) ? hist[med - d] : 0u;
+        whist[0][d] = up + dn;
+    }
     __syncthreads();
     if (tid < 64) {
-        const int m = hist_rank(hist, rank, lane);
+        const int m = hist_rank(whist[0], rank, lane);
         if (lane == 0) sh_mad = m;
     }
     __syncthreads();

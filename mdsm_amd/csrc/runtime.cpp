@@ -168,6 +168,7 @@ struct mdsm_server {
     std::deque<Done> done;
     std::thread sched;
     bool stop = false;
+    bool held = false;  // admissions paused (mdsm_server_hold)
     hipStream_t streams[kStreams] = {nullptr, nullptr};
     uint32_t *counters = nullptr;  // persistent-mode work counters, one per stream
     int next_stream = 0;
@@ -425,7 +426,7 @@ void scheduler(mdsm_server *s)
         if (progress) s->cv_done.notify_all();
         // admit a batch
         Batch *bt = nullptr;
-        if (s->inflight.size() < (size_t)kStreams * 2) {
+        if (!s->held && s->inflight.size() < (size_t)kStreams * 2) {
             std::vector<Beam *> order;
             for (auto &kv : s->beams) order.push_back(kv.second);
             const size_t nb = order.size();
@@ -822,10 +823,22 @@ extern "C" int mdsm_server_release(mdsm_server *s, uint32_t id, uint64_t seq)
     return MDSM_OK;
 }
 
+extern "C" int mdsm_server_hold(mdsm_server *s, int hold)
+{
+    if (!s) return fail(MDSM_ERR_INVALID, "NULL argument");
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        s->held = hold != 0;
+    }
+    s->cv_work.notify_all();
+    return MDSM_OK;
+}
+
 extern "C" int mdsm_server_drain(mdsm_server *s)
 {
     if (!s) return fail(MDSM_ERR_INVALID, "NULL argument");
     std::unique_lock<std::mutex> lk(s->mu);
+    s->held = false;  // a held server would never drain
     for (;;) {
         bool idle = s->inflight.empty();
         for (auto &kv : s->beams) idle = idle && kv.second->queue.empty() && !kv.second->busy;

@@ -64,6 +64,7 @@ def _sig():
         "mdsm_server_poll": [vp, C.POINTER(Result), vp, C.c_uint32, C.c_int32],
         "mdsm_server_output": [vp, C.c_uint32, C.c_uint64, C.POINTER(vp)],
         "mdsm_server_release": [vp, C.c_uint32, C.c_uint64],
+        "mdsm_server_hold": [vp, C.c_int],
         "mdsm_server_drain": [vp],
         "mdsm_server_get_stats": [vp, C.POINTER(ServerStats)],
     }
@@ -185,6 +186,11 @@ class Server:
 
     def release(self, b: Beam, seq: int):
         check(_sig().mdsm_server_release(self.h, b.id, seq))
+
+    def hold(self, flag=True):
+        """Pause (True) or resume (False) admissions: blocks pushed while held
+        are admitted together on resume."""
+        check(_sig().mdsm_server_hold(self.h, int(bool(flag))))
 
     def drain(self):
         check(_sig().mdsm_server_drain(self.h))

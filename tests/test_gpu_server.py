@@ -24,8 +24,10 @@ def _stream(M, C, N, nblocks, seed, pulses):
                               fch1=1500.0, foff=-300.0 / C, pulses=pulses)
 
 
-def _run(srv, beams_and_streams, N, nblocks, interleave=True):
-    """Push every block of every beam; returns {(beam, seq): (out, cands)}."""
+def _run(srv, beams_and_streams, N, nblocks, interleave=True, hold_first=False):
+    """Push every block of every beam; returns {(beam, seq): (out, cands)}.
+    hold_first: admissions are held while every beam's first block is pushed,
+    so that those blocks are admitted as one batch."""
     res = {}
     pending = 0
     order = []
@@ -34,6 +36,13 @@ def _run(srv, beams_and_streams, N, nblocks, interleave=True):
             order.append((b, k, x))
     if not interleave:
         order.sort(key=lambda t: (t[0].id, t[1]))
+    if hold_first:
+        srv.hold(True)
+        for b, k, x in order[:len(beams_and_streams)]:
+            assert srv.try_push(b, np.ascontiguousarray(x[:, k * N:(k + 1) * N])) is not None
+            pending += 1
+        order = order[len(beams_and_streams):]
+        srv.hold(False)
     for b, k, x in order:
         blk = np.ascontiguousarray(x[:, k * N:(k + 1) * N])
         while srv.try_push(b, blk) is None:
@@ -106,7 +115,7 @@ def test_alone_and_in_mix_identical_and_match_oracle(gpu):
     # mix: all beams on one server, interleaved pushes, batched launches
     srv = Server(max_batch=8)
     bs = [srv.open_beam(**_beam_kwargs(C, N, ranges)) for _ in streams]
-    mix = _run(srv, list(zip(bs, streams)), N, nblocks)
+    mix = _run(srv, list(zip(bs, streams)), N, nblocks, hold_first=True)
     batches = [v[2].batch for v in mix.values()]
     assert max(batches) > 1, "blocks were never batched together"
     for i, b in enumerate(bs):
@@ -139,7 +148,7 @@ def test_budget_admission(gpu):
     C, N = 64, 4096
     bs = [srv.open_beam(**_beam_kwargs(C, N)) for _ in range(3)]
     xs = [_stream(M, C, N, 2, 7 + i, []) for i in range(3)]
-    out = _run(srv, list(zip(bs, xs)), N, 2)
+    out = _run(srv, list(zip(bs, xs)), N, 2, hold_first=True)
     assert len(out) == 6
     st = srv.stats()
     assert st["deferred_inflight"] > 0 and st["bytes_peak"] <= st["budget"]
