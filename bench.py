@@ -650,7 +650,10 @@ def main():
         "valu": {"adds_per_launch": adds_per_launch,
                  "achieved_tadds": adds_per_launch / avg_launch_s / 1e12,
                  "peak_tadds": VALU_ADD_PEAK_T,
-                 "frac": adds_per_launch / avg_launch_s / 1e12 / VALU_ADD_PEAK_T},
+                 "frac": adds_per_launch / avg_launch_s / 1e12 / VALU_ADD_PEAK_T,
+                 "note": "one add per DM*chan*sample against one 32-bit add per lane per cycle; "
+                         "uint8 rows add two channels of two samples per v_add3_u32 (packed "
+                         "16-bit lanes), so the VALU issue rate is below this fraction"},
         "kernels_ms_per_step": {"dedisperse": ded_ms / args.steps,
                                 "decimate": dec_ms / args.steps},
     }
