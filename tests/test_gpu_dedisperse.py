@@ -92,6 +92,25 @@ def test_multi_range_decimation_u8(gpu):
         _check_range(plan, r, x, outs[r], "u8")
 
 
+@pytest.mark.parametrize("chunk_rows", [2, 3, 5, 7, 8])
+def test_u8_chunk_shapes_exact(gpu, chunk_rows):
+    """uint8 and decimated uint16 rows with every chunk shape the offset
+    registers meet: odd chunks (a lone last channel), a channel count that
+    is no multiple of the chunk, partial trial blocks; bit-exact."""
+    M = gpu
+    C, S = 75, 9000
+    print(f"case u8_chunk_shapes chunk_rows={chunk_rows}", flush=True)
+    x = M.synth_filterbank(C, S, "u8", seed=23, mean=128, sigma=16, foff=-300.0 / C,
+                           pulses=[(60.0, 4000, 4, 4.0)])
+    plan = _plan(M, C, S, "u8", [(0.0, 0.75, 100, 1), (75.0, 1.5, 41, 2)], foff=-300.0 / C)
+    t = plan.tuning()
+    t["chunk_rows"] = chunk_rows
+    plan.set_tuning(**t)
+    outs = plan.dedisperse_host(x)
+    for r in range(2):
+        _check_range(plan, r, x, outs[r], "u8")
+
+
 @pytest.mark.parametrize("pad", [0, 5])
 def test_decimation_pyramid_u8(gpu, pad):
     """uint8 ranges at k = 2, 4, 8, 16 (one fused decimation pass) and k = 3
