@@ -47,6 +47,31 @@ def test_single_range_small(gpu, dtype):
     _check_range(plan, 0, x, outs[0], dtype)
 
 
+@pytest.mark.parametrize("C,S,dtype,ranges", [
+    (1, 17, "u8", [(0.0, 1.0, 1, 1)]),                     # one channel, one trial, one short tile
+    (2, 300, "f32", [(0.0, 3.0, 1, 1)]),
+    (3, 257, "f16", [(0.0, 2.0, 5, 2)]),                   # decimated, 126 outputs
+    (7, 1000, "u8", [(0.0, 1.0, 65, 1)]),                  # a second block holding one trial
+    (1, 5000, "f32", [(0.0, 1.0, 3, 4)]),
+    (5, 700, "u8", [(0.0, 0.5, 9, 1), (4.5, 1.0, 3, 16)]),  # 16x decimated uint16 rows, 43 outputs
+])
+def test_tiny_shapes(gpu, C, S, dtype, ranges):
+    """Shapes below one tile, one channel chunk and one trial block."""
+    M = gpu
+    print(f"case tiny C={C} S={S} dtype={dtype} ranges={ranges}", flush=True)
+    rng = np.random.default_rng(C * 1000 + S)
+    if dtype == "u8":
+        x = rng.integers(0, 256, (C, S), dtype=np.uint8)
+    else:
+        x = rng.standard_normal((C, S)).astype(np.float16 if dtype == "f16" else np.float32)
+    foff = -25.0 / C
+    plan = _plan(M, C, S, dtype, ranges, foff=foff)
+    outs = plan.dedisperse_host(x)
+    for r in range(len(ranges)):
+        assert outs[r].shape == (ranges[r][2], plan.ranges[r].nout)
+        _check_range(plan, r, x, outs[r], dtype)
+
+
 def test_const_per_channel_kat(gpu):
     """x[c][t] = c gives every output C(C-1)/2 (src/prototypes/brute_dedisp_imp_tex.cu:112-114,167-175)."""
     M = gpu
