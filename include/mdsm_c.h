@@ -37,7 +37,8 @@ typedef enum {
     MDSM_ERR_BUDGET = -5,   /* request can never fit the HBM budget */
     MDSM_ERR_STATE = -6,    /* call out of order / object closed */
     MDSM_ERR_PARSE = -7,    /* observation XML could not be parsed */
-    MDSM_ERR_AGAIN = -8     /* nothing ready yet / admission deferred */
+    MDSM_ERR_AGAIN = -8,    /* nothing ready yet / admission deferred */
+    MDSM_ERR_CANCELLED = -9 /* block dropped by mdsm_beam_cancel before it ran */
 } mdsm_status;
 
 /* Sample storage of the filterbank.  uint8 is the plain byte value
@@ -326,7 +327,9 @@ int mdsm_clip_u8(const uint8_t *d_in, int64_t in_stride, uint32_t nchans,
 
 typedef struct {
     uint32_t dm;     /* output row */
-    uint32_t t;      /* first sample of the boxcar */
+    int32_t t;       /* first sample of the boxcar, relative to the block's first
+                        sample (negative in a stream search: the window starts
+                        in the previous block) */
     uint32_t width;  /* boxcar width in samples */
     float snr;
 } mdsm_cand;
@@ -349,6 +352,28 @@ int mdsm_detect_boxcar(const float *d_x, uint32_t ndms, uint32_t n,
                        double thr, mdsm_cand *d_cands, uint32_t cap,
                        uint32_t *d_count, double *d_mean, double *d_sigma,
                        void *stream);
+
+/* Streaming form of mdsm_detect_boxcar for consecutive blocks of one
+ * series: every boxcar window is searched exactly once, windows that
+ * straddle two blocks included.  d_carry ([ndms][carry_stride] floats, at
+ * least mdsm_boxcar_carry_len(widths) = maxw + 1 rounded up to a multiple of
+ * 4 per row) holds the previous block's last samples; carry_valid = 0 for the
+ * first block of a stream (nothing carried).  The searched series of row d is
+ * the carried samples followed by the block's n samples; window (t, w) is
+ * searched by the block holding sample t + w, the last one its right
+ * neighbour reads (t >= -carried), with this block's mean and sigma (over its
+ * n samples; d_mean/d_sigma receive them).  Candidate t is relative to the
+ * block's first sample.  The peak rule is mdsm_detect_boxcar's; the window
+ * ending at the block's last sample is left to the next block.  The call then
+ * stores the block's last samples in d_carry (requires n >= the carry
+ * length).  No counterpart in the reference, whose detectors search each
+ * block on its own (src/mdsm/dedispersion_output.cpp:176-181). */
+uint32_t mdsm_boxcar_carry_len(uint32_t widths_mask);
+int mdsm_detect_boxcar_stream(const float *d_x, uint32_t ndms, uint32_t n, int64_t stride,
+                              float *d_carry, int64_t carry_stride, int32_t carry_valid,
+                              const uint32_t *widths, uint32_t nwidths, double thr,
+                              mdsm_cand *d_cands, uint32_t cap, uint32_t *d_count,
+                              double *d_mean, double *d_sigma, void *stream);
 
 /* Median-of-five threshold detector of the reference's output thread
  * (src/mdsm/dedispersion_output.cpp:14-53,56-213): per row d, for l < n-4,
@@ -435,13 +460,27 @@ typedef struct {
     uint32_t pad;
 } mdsm_detect_config;
 
+/* mdsm_result.flags */
+enum {
+    MDSM_RESULT_OVERFLOW = 1,    /* a range found more than cand_cap candidates: the kept
+                                    subset of that range is arbitrary (atomic order) */
+    MDSM_RESULT_ZERO_FILLED = 2, /* the block never arrived (a sequence gap) and was
+                                    processed as zeros, as the reference's UDP chunker
+                                    fills lost packets (src/standalone/UdpChunker.cpp:
+                                    104-128) */
+    MDSM_RESULT_SKIPPED = 4      /* zero-filled block whose whole history is zeros: its
+                                    output is exactly zero and was not computed */
+};
+
 typedef struct {
     uint32_t beam;
     int32_t status;         /* MDSM_OK, or the error that ended the block */
     uint64_t seq;           /* block number within the beam */
-    uint32_t ncand;         /* candidates found (may exceed those returned) */
+    uint32_t ncand;         /* candidates found (may exceed those kept and returned) */
+    uint32_t nreturned;     /* candidates copied to poll's cands (<= its cap) */
     uint32_t nclipped;      /* samples replaced by the clipper */
     uint32_t batch;         /* blocks in the launch that carried this block */
+    uint32_t flags;         /* MDSM_RESULT_* */
     uint32_t pad;
     double t_arrival;       /* seconds, steady clock (as passed to push, or push time) */
     double t_admit;         /* admitted to the device */
@@ -459,22 +498,57 @@ void mdsm_server_destroy(mdsm_server *srv);
 
 /* Open a beam.  obs->nsamp_in is N, the new samples per block.  The beam's
  * overlap is M = the plan's maxshift (full-rate samples); output sample t of
- * block k (range r, decimation k_r) starts at stream sample k*N - M + t*k_r.
- * Detection runs on the first N/k_r samples of every range row.
+ * block k (range r, decimation k_r) starts at stream sample k*N - M + t*k_r,
+ * so the first n_r = N/k_r samples of each row of consecutive blocks tile the
+ * range's series without gap or repeat.  Detection searches that series as
+ * mdsm_detect_boxcar_stream does: each beam keeps the last
+ * mdsm_boxcar_carry_len(widths) samples of every DM row resident on the
+ * device, so boxcar windows straddling two blocks are searched once (by the
+ * later block; candidate t may be negative), with statistics per row and
+ * block.  Requires n_r >= that carry length.
  * MDSM_ERR_BUDGET when the beam's resident buffers plus one block can never
  * fit the budget. */
 int mdsm_beam_open(mdsm_server *srv, const mdsm_obs *obs, const mdsm_range *ranges,
                    uint32_t nranges, const mdsm_detect_config *det, uint32_t *beam_id);
+/* Close a beam: every block already pushed is processed and reported first
+ * (also while admissions are held), then its resources are freed. */
 int mdsm_beam_close(mdsm_server *srv, uint32_t beam_id);
+/* Close a beam mid-stream: its queued blocks are dropped and reported with
+ * status MDSM_ERR_CANCELLED; a block already on the device completes and is
+ * reported as usual. */
+int mdsm_beam_cancel(mdsm_server *srv, uint32_t beam_id);
 /* overlap M, plan info of the beam (nsamp_in of the plan is M + N) */
 int mdsm_beam_info(mdsm_server *srv, uint32_t beam_id, uint32_t *overlap,
                    mdsm_plan_info *info);
 
 /* Queue one block: host channel-major [nchans][stride] samples of obs.dtype,
  * N per channel; copied before return.  arrival_s < 0: stamped now.
+ * *seq receives the block's number (the next in the beam's sequence).
  * MDSM_ERR_AGAIN when the beam's queue is full (back-pressure). */
 int mdsm_beam_push(mdsm_server *srv, uint32_t beam_id, const void *host, int64_t stride,
                    double arrival_s, uint64_t *seq);
+
+/* Push with the caller's block number, as a packet stream numbers its
+ * blocks.  seq equal to the beam's next expected number: as mdsm_beam_push.
+ * seq below it (a duplicate or a block that arrives after its gap was
+ * filled): MDSM_ERR_STATE, nothing queued.  seq above it: each missing block
+ * is queued first as a block of zeros and reported with
+ * MDSM_RESULT_ZERO_FILLED (the reference's chunker fills lost packets with
+ * empty ones and drops duplicates, src/standalone/UdpChunker.cpp:100-128);
+ * missing blocks whose whole history is zeros (more than M/N of them in a
+ * row) are not computed -- their output is exactly zero -- and are reported
+ * with MDSM_RESULT_SKIPPED as well.  Missing blocks do not count against the
+ * queue depth; gaps longer than 2^20 blocks are MDSM_ERR_INVALID. */
+int mdsm_beam_push_seq(mdsm_server *srv, uint32_t beam_id, const void *host, int64_t stride,
+                       uint64_t seq, double arrival_s);
+
+/* Queue one block that is already in device memory (channel-major
+ * [nchans][stride] samples of obs.dtype on the server's device), for callers
+ * whose upstream stage runs on the GPU.  The server reads d_block when the
+ * block runs and does not copy it first: it must stay allocated and unchanged
+ * until the block's result has been polled. */
+int mdsm_beam_push_device(mdsm_server *srv, uint32_t beam_id, const void *d_block,
+                          int64_t stride, double arrival_s, uint64_t *seq);
 
 /* Zero-copy push: mdsm_beam_acquire lends the beam's next free pinned
  * staging block ([nchans][N] samples of obs.dtype, row stride N); the caller
@@ -487,9 +561,10 @@ int mdsm_beam_acquire(mdsm_server *srv, uint32_t beam_id, void **host_block);
 int mdsm_beam_commit(mdsm_server *srv, uint32_t beam_id, void *host_block,
                      double arrival_s, uint64_t *seq);
 
-/* Next completed block.  Its candidates (sorted by dm, width, t) are copied
- * into cands[0..min(ncand,cap)).  wait_ms < 0 waits indefinitely.
- * MDSM_ERR_AGAIN when nothing completed in time. */
+/* Next completed block.  Its kept candidates (sorted by dm, width, t; at
+ * most cand_cap per range) are copied into cands[0..res->nreturned),
+ * nreturned <= cap.  wait_ms < 0 waits indefinitely.  MDSM_ERR_AGAIN when
+ * nothing completed in time. */
 int mdsm_server_poll(mdsm_server *srv, mdsm_result *res, mdsm_cand *cands, uint32_t cap,
                      int32_t wait_ms);
 
@@ -508,6 +583,21 @@ int mdsm_server_hold(mdsm_server *srv, int hold);
 /* Block until every pushed block has completed (results stay queued).
  * Resumes admissions if they were held. */
 int mdsm_server_drain(mdsm_server *srv);
+
+/* Beam state for re-admission: the device-resident overlap (the last M input
+ * samples of every channel, after clipping), the detector's carried samples
+ * of every DM row and the next sequence number.  mdsm_beam_export waits until
+ * the beam's pushed blocks have completed (admitting them while held), then
+ * writes the state to host (*bytes = its size; with host NULL or cap too
+ * small only *bytes is set and MDSM_ERR_CAPACITY returned).
+ * mdsm_beam_import loads it into a beam opened with the same observation,
+ * plan and widths before any block was pushed to it; the stream then goes
+ * on as if it had never left.  The reference carries the overlap on the host
+ * (src/mdsm/dedispersion_thread.cu:266-303) and the per-chunk noise
+ * statistics in the module (src/pelican-mdsm/modules/MdsmModule.cpp:226-237). */
+int mdsm_beam_export(mdsm_server *srv, uint32_t beam_id, void *host, uint64_t cap,
+                     uint64_t *bytes);
+int mdsm_beam_import(mdsm_server *srv, uint32_t beam_id, const void *host, uint64_t bytes);
 int mdsm_server_get_stats(mdsm_server *srv, mdsm_server_stats *st);
 /* Steady clock in seconds (the clock of t_arrival/t_admit/t_done). */
 double mdsm_now(void);

@@ -104,11 +104,11 @@ class RangeInfo(C.Structure):
 
 
 class Cand(C.Structure):
-    _fields_ = [("dm", C.c_uint32), ("t", C.c_uint32), ("width", C.c_uint32),
+    _fields_ = [("dm", C.c_uint32), ("t", C.c_int32), ("width", C.c_uint32),
                 ("snr", C.c_float)]
 
 
-CAND_DTYPE = np.dtype([("dm", np.uint32), ("t", np.uint32), ("width", np.uint32),
+CAND_DTYPE = np.dtype([("dm", np.uint32), ("t", np.int32), ("width", np.uint32),
                        ("snr", np.float32)])
 
 
@@ -159,6 +159,8 @@ def lib():
         "mdsm_decimate": [vp, i32, u32, i64, u32, u32, vp, i64, vp],
         "mdsm_clip_u8": [vp, i64, u32, i64, u32, f32, vp, i64, vp, vp, vp, vp],
         "mdsm_detect_boxcar": [vp, u32, u32, i64, vp, u32, f64, vp, u32, vp, vp, vp, vp],
+        "mdsm_detect_boxcar_stream": [vp, u32, u32, i64, vp, i64, i32, vp, u32, f64, vp, u32, vp,
+                                      vp, vp, vp],
         "mdsm_detect_median5": [vp, u32, u32, i64, vp, f64, i32, vp, vp, u32, vp, u32, vp, vp,
                                 vp],
         "mdsm_synth_filterbank": [C.POINTER(SynthSpec), vp, i64],
@@ -212,6 +214,8 @@ def lib():
     for name in ("mdsm_sigproc_close", "mdsm_series_writer_close"):
         getattr(L, name).argtypes = [vp]
         getattr(L, name).restype = None
+    L.mdsm_boxcar_carry_len.restype = C.c_uint32
+    L.mdsm_boxcar_carry_len.argtypes = [u32]
     L.mdsm_last_error.restype = C.c_char_p
     L.mdsm_version.restype = C.c_char_p
     _lib = L
@@ -650,6 +654,27 @@ def detect_boxcar(d_x, ndms, n, stride, widths, thr, d_cands, cap, d_count, d_me
     w = (C.c_uint32 * len(widths))(*widths)
     check(lib().mdsm_detect_boxcar(g(d_x), ndms, n, stride, w, len(widths), thr, g(d_cands), cap,
                                    g(d_count), g(d_mean), g(d_sigma), _sh(stream)))
+
+
+def boxcar_carry_len(widths) -> int:
+    """Carried samples per DM row of a streaming search with these widths."""
+    mask = 0
+    for w in widths:
+        mask |= int(w)
+    return int(lib().mdsm_boxcar_carry_len(mask))
+
+
+def detect_boxcar_stream(d_x, ndms, n, stride, d_carry, carry_stride, carry_valid, widths, thr,
+                         d_cands, cap, d_count, d_mean, d_sigma, stream=None):
+    """Streaming boxcar search of one block (mdsm_detect_boxcar_stream): the
+    carry [ndms][carry_stride] holds the previous block's last samples and is
+    updated with this block's."""
+    g = lambda b: b.ptr if isinstance(b, DeviceBuffer) else b
+    w = (C.c_uint32 * len(widths))(*widths)
+    check(lib().mdsm_detect_boxcar_stream(g(d_x), ndms, n, stride, g(d_carry), carry_stride,
+                                          1 if carry_valid else 0, w, len(widths), thr,
+                                          g(d_cands), cap, g(d_count), g(d_mean), g(d_sigma),
+                                          _sh(stream)))
 
 
 def detect_median5(d_x, ndms, n, stride, d_dmv, thr, d_cands, cap, d_count, d_stats,

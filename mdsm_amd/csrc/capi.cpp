@@ -187,6 +187,50 @@ extern "C" int mdsm_detect_boxcar(const float *d_x, uint32_t ndms, uint32_t n, i
     return MDSM_OK;
 }
 
+extern "C" uint32_t mdsm_boxcar_carry_len(uint32_t widths_mask)
+{
+    return boxcar_carry_len(widths_mask);
+}
+
+extern "C" int mdsm_detect_boxcar_stream(const float *d_x, uint32_t ndms, uint32_t n,
+                                         int64_t stride, float *d_carry, int64_t carry_stride,
+                                         int32_t carry_valid, const uint32_t *widths,
+                                         uint32_t nwidths, double thr, mdsm_cand *d_cands,
+                                         uint32_t cap, uint32_t *d_count, double *d_mean,
+                                         double *d_sigma, void *stream)
+{
+    if (!d_x || !d_carry || !widths || !d_cands || !d_count || !d_mean || !d_sigma)
+        return fail(MDSM_ERR_INVALID, "NULL argument");
+    if (stride < (int64_t)n || n < 1) return fail(MDSM_ERR_INVALID, "bad row length/stride");
+    if (ndms > 65535) return fail(MDSM_ERR_INVALID, "ndms > 65535");
+    uint32_t mask = 0;
+    for (uint32_t i = 0; i < nwidths; ++i) {
+        uint32_t w = widths[i];
+        if (w == 0 || (w & (w - 1)) || w > 1024)
+            return fail(MDSM_ERR_INVALID, "boxcar widths must be powers of two <= 1024");
+        mask |= w;
+    }
+    if (!mask) return fail(MDSM_ERR_INVALID, "no widths");
+    const uint32_t H = boxcar_carry_len(mask);
+    if (carry_stride < (int64_t)H) return fail(MDSM_ERR_INVALID, "carry_stride < carry length");
+    if (n < H) return fail(MDSM_ERR_INVALID, "block shorter than the carried samples");
+    SqrtW sw;
+    for (int i = 0; i < 11; ++i) sw.v[i] = std::sqrt((double)(1u << i));
+    BoxStream bs{d_carry, carry_stride, carry_valid ? (int32_t)H : 0, 1};
+    hipStream_t st = (hipStream_t)stream;
+    {
+        TimedLaunch tl("detect", st);
+        hipError_t e = launch_boxcar_stream(d_x, ndms, n, stride, bs, mask, thr, sw, d_cands, cap,
+                                            d_count, d_mean, d_sigma, true, st);
+        if (e != hipSuccess) return hip_fail(e, "detect launch");
+    }
+    // the block's last H samples are the next block's carry
+    hipError_t e = launch_copy2d(d_carry, carry_stride * 4, d_x + (n - H), stride * 4,
+                                 (int64_t)H * 4, ndms, st);
+    if (e != hipSuccess) return hip_fail(e, "carry save");
+    return MDSM_OK;
+}
+
 extern "C" int mdsm_detect_median5(const float *d_x, uint32_t ndms, uint32_t n, int64_t stride,
                                    const float *d_dm_values, double thr, int32_t per_row_stats,
                                    const float *d_chunk_mean, const float *d_chunk_rms,

@@ -75,6 +75,24 @@ hipError_t launch_copy2d(void *dst, int64_t dpitch, const void *src, int64_t spi
     return hipGetLastError();
 }
 
+// 2D zero fill (rows of `width` bytes), a kernel so that it can be captured.
+__global__ void zero2d_kernel(uint8_t *dst, int64_t dpitch, uint32_t width, uint32_t height)
+{
+    for (int64_t r = blockIdx.y; r < height; r += gridDim.y)
+        for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < width; i += gridDim.x * 256u)
+            dst[r * dpitch + i] = 0;
+}
+
+hipError_t launch_zero2d(void *dst, int64_t dpitch, int64_t width, uint32_t height, hipStream_t s)
+{
+    if (width <= 0 || height == 0) return hipSuccess;
+    if (width > 0xffffffffLL) return hipErrorInvalidValue;
+    const uint32_t bx = (uint32_t)std::min<int64_t>((width + 255) / 256, 1024);
+    hipLaunchKernelGGL(zero2d_kernel, dim3(bx, std::min<uint32_t>(height, 65535u)), dim3(256), 0,
+                       s, (uint8_t *)dst, dpitch, (uint32_t)width, height);
+    return hipGetLastError();
+}
+
 // Plain single-beam float32 dedispersion in the reference GPU kernel's shape
 // (src/mdsm/dedispersion_kernel.cu:51-77, launched at
 // src/mdsm/dedispersion_thread.cu:229-231): one thread per (DM, output
@@ -1306,8 +1324,8 @@ constexpr int kMaxBoxW = 1024;      // widest boxcar
 constexpr int kBoxL = kBT + kMaxBoxW + 1;
 
 __global__ void __launch_bounds__(256)
-    boxcar_kernel(const float *x, uint32_t n, int64_t stride, uint32_t wmask, uint32_t maxw,
-                  double thr, SqrtW sqrtw, mdsm_cand *cands, uint32_t cap,
+    boxcar_kernel(const float *x, uint32_t n, int64_t stride, BoxStream bs, uint32_t wmask,
+                  uint32_t maxw, double thr, SqrtW sqrtw, mdsm_cand *cands, uint32_t cap,
                   uint32_t *count, const double *mean, const double *sigma)
 {
     __shared__ double buf[2][kBoxL];
@@ -1317,11 +1335,13 @@ __global__ void __launch_bounds__(256)
     if (!(sg > 0.0)) return;
     const double mu = mean[d];
     const float *row = x + (int64_t)d * stride;
+    const float *crow = bs.carry + (int64_t)d * bs.carry_stride;
+    const int64_t H = bs.H, ne = (int64_t)n + H;        // extended series length
     const int64_t b = (int64_t)blockIdx.x * kBT - 1;     // sample at local index 0
     const int L = kBT + (int)maxw + 1;
     for (int u = tid; u < L; u += 256) {
         const int64_t t = b + u;
-        buf[0][u] = (t >= 0 && t < (int64_t)n) ? (double)row[t] : 0.0;
+        buf[0][u] = (t >= 0 && t < ne) ? (double)(t < H ? crow[t] : row[t - H]) : 0.0;
     }
     __syncthreads();
     int cur = 0;
@@ -1334,10 +1354,12 @@ __global__ void __launch_bounds__(256)
             // that bound skip the division without changing any decision
             const double td = thr * den;
             const double lo = td - fabs(td) * 0x1p-40;
-            const int64_t last = (int64_t)n - w;
+            const int64_t last = ne - w;
+            const int64_t tmin = H > (int64_t)w ? H - w : 0;
             for (int u = 1 + tid; u <= kBT; u += 256) {
                 const int64_t t = b + u;
-                if (t > last) break;
+                if (t > last - bs.tail) break;
+                if (t < tmin) continue;
                 const double sv = buf[cur][u];
                 const double num = sv - wm;
                 if (!(num > lo)) continue;
@@ -1349,7 +1371,7 @@ __global__ void __launch_bounds__(256)
                 if (slot < cap) {
                     mdsm_cand cd;
                     cd.dm = d;
-                    cd.t = (uint32_t)t;
+                    cd.t = (int32_t)(t - H);
                     cd.width = w;
                     cd.snr = (float)snr;
                     cands[slot] = cd;
@@ -1390,9 +1412,9 @@ __device__ __forceinline__ double next_lane_d(double v)
 }
 
 __global__ void __launch_bounds__(256)
-    boxcar_reg_kernel(const float *x, uint32_t n, int64_t stride, uint32_t wmask, uint32_t maxw,
-                      double thr, SqrtW sqrtw, mdsm_cand *cands, uint32_t cap, uint32_t *count,
-                      const double *mean, const double *sigma)
+    boxcar_reg_kernel(const float *x, uint32_t n, int64_t stride, BoxStream bs, uint32_t wmask,
+                      uint32_t maxw, double thr, SqrtW sqrtw, mdsm_cand *cands, uint32_t cap,
+                      uint32_t *count, const double *mean, const double *sigma)
 {
     const uint32_t d = blockIdx.y;
     const double sg = sigma[d];
@@ -1400,16 +1422,24 @@ __global__ void __launch_bounds__(256)
     const double mu = mean[d];
     const int lane = threadIdx.x & 63;
     const int wave = threadIdx.x >> 6;
+    const int64_t H = bs.H, ne = (int64_t)n + H;  // extended series: carry, then the row
     const int64_t T0 = ((int64_t)blockIdx.x * 4 + wave) * kBR;
-    if (T0 >= (int64_t)n) return;  // the whole wave
+    if (T0 >= ne) return;  // the whole wave
     const float *row = x + (int64_t)d * stride;
+    const float *crow = bs.carry + (int64_t)d * bs.carry_stride;
     const int64_t b = T0 - 4;       // sample held by lane 0, slot 0
     const int64_t tl = b + kBL * lane;
     double s[kBL];
-    if (tl >= 0 && tl + kBL <= (int64_t)n && (((uintptr_t)(row + tl)) & 15) == 0) {
+    // 16-byte loads when every float4 of the lane's 16 samples is aligned and
+    // lies in one source (H, tl and both row bases are multiples of 4 floats)
+    const bool vec = tl >= 0 && tl + kBL <= ne && ((tl | H) & 3) == 0 &&
+                     (((uintptr_t)row | (uintptr_t)crow | ((uint64_t)stride << 2) |
+                       ((uint64_t)bs.carry_stride << 2)) & 15) == 0;
+    if (vec) {
 #pragma unroll
         for (int q = 0; q < kBL / 4; ++q) {
-            const float4 v = ((const float4 *)(row + tl))[q];
+            const int64_t e = tl + 4 * q;
+            const float4 v = e < H ? *(const float4 *)(crow + e) : *(const float4 *)(row + e - H);
             s[4 * q] = v.x;
             s[4 * q + 1] = v.y;
             s[4 * q + 2] = v.z;
@@ -1419,7 +1449,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int j = 0; j < kBL; ++j) {
             const int64_t t = tl + j;
-            s[j] = (t >= 0 && t < (int64_t)n) ? (double)row[t] : 0.0;
+            s[j] = (t >= 0 && t < ne) ? (double)(t < H ? crow[t] : row[t - H]) : 0.0;
         }
     }
 #pragma unroll
@@ -1431,14 +1461,16 @@ __global__ void __launch_bounds__(256)
             const double den = sg * sqrtw.v[lv];
             const double td = thr * den;
             const double lo = td - fabs(td) * 0x1p-40;  // exact pre-filter, as above
-            const int64_t last = (int64_t)n - w;
+            const int64_t last = ne - w;
+            const int64_t tmax = last - bs.tail;
+            const int64_t tmin = H > (int64_t)w ? H - w : 0;
             bool c[kBL];
             bool any = false;
 #pragma unroll
             for (int j = 0; j < kBL; ++j) {
                 const int u = kBL * lane + j;
                 const int64_t t = b + u;
-                c[j] = u >= 4 && u < 4 + kBR && t <= last && (s[j] - wm) > lo;
+                c[j] = u >= 4 && u < 4 + kBR && t <= tmax && t >= tmin && (s[j] - wm) > lo;
                 any = any || c[j];
             }
             if (__any(any)) {
@@ -1459,7 +1491,7 @@ __global__ void __launch_bounds__(256)
                     if (slot < cap) {
                         mdsm_cand cd;
                         cd.dm = d;
-                        cd.t = (uint32_t)t;
+                        cd.t = (int32_t)(t - H);
                         cd.width = w;
                         cd.snr = (float)snr;
                         cands[slot] = cd;
@@ -1488,27 +1520,54 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+uint32_t boxcar_maxw(uint32_t wmask)
+{
+    uint32_t maxw = 1;
+    while ((maxw << 1) <= wmask && (maxw << 1) <= (uint32_t)kMaxBoxW) maxw <<= 1;
+    return maxw;
+}
+
+uint32_t boxcar_carry_len(uint32_t wmask)
+{
+    return (boxcar_maxw(wmask) + 1 + 3) & ~3u;
+}
+
+hipError_t launch_boxcar_stream(const float *x, uint32_t ndms, uint32_t n, int64_t stride,
+                                const BoxStream &bs, uint32_t wmask, double thr,
+                                const SqrtW &sqrtw, mdsm_cand *cands, uint32_t cap,
+                                uint32_t *count, double *mean, double *sigma, bool stats,
+                                hipStream_t s)
+{
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, count, 1u, 0u);
+    if (ndms == 0 || n == 0) return hipGetLastError();
+    const uint32_t maxw = boxcar_maxw(wmask);
+    if (stats) {
+        hipLaunchKernelGGL(row_sums_kernel, dim3(ndms), dim3(256), 0, s, x, n, stride, mean,
+                           sigma);
+        hipLaunchKernelGGL(row_stats_finish, dim3((ndms + 255) / 256), dim3(256), 0, s, ndms, n,
+                           mean, sigma);
+    }
+    const int64_t ne = (int64_t)n + bs.H;
+    if (maxw <= 64) {
+        const uint32_t tiles = (uint32_t)((ne + kBR - 1) / kBR);
+        hipLaunchKernelGGL(boxcar_reg_kernel, dim3((tiles + 3) / 4, ndms), dim3(256), 0, s, x, n,
+                           stride, bs, wmask, maxw, thr, sqrtw, cands, cap, count, mean, sigma);
+        return hipGetLastError();
+    }
+    dim3 grid((uint32_t)((ne + kBT - 1) / kBT), ndms);
+    hipLaunchKernelGGL(boxcar_kernel, grid, dim3(256), 0, s, x, n, stride, bs, wmask, maxw, thr,
+                       sqrtw, cands, cap, count, mean, sigma);
+    return hipGetLastError();
+}
+
 hipError_t launch_boxcar(const float *x, uint32_t ndms, uint32_t n, int64_t stride, uint32_t wmask,
                          double thr, const SqrtW &sqrtw, mdsm_cand *cands, uint32_t cap,
                          uint32_t *count, double *mean, double *sigma, hipStream_t s)
 {
-    if (ndms == 0 || n == 0) return hipSuccess;
-    uint32_t maxw = 1;
-    while ((maxw << 1) <= wmask && (maxw << 1) <= (uint32_t)kMaxBoxW) maxw <<= 1;
-    hipLaunchKernelGGL(row_sums_kernel, dim3(ndms), dim3(256), 0, s, x, n, stride, mean, sigma);
-    hipLaunchKernelGGL(row_stats_finish, dim3((ndms + 255) / 256), dim3(256), 0, s, ndms, n, mean,
-                       sigma);
-    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, count, 1u, 0u);
-    if (maxw <= 64) {
-        const uint32_t tiles = (n + kBR - 1) / kBR;
-        hipLaunchKernelGGL(boxcar_reg_kernel, dim3((tiles + 3) / 4, ndms), dim3(256), 0, s, x, n,
-                           stride, wmask, maxw, thr, sqrtw, cands, cap, count, mean, sigma);
-        return hipGetLastError();
-    }
-    dim3 grid((n + kBT - 1) / kBT, ndms);
-    hipLaunchKernelGGL(boxcar_kernel, grid, dim3(256), 0, s, x, n, stride, wmask, maxw, thr, sqrtw,
-                       cands, cap, count, mean, sigma);
-    return hipGetLastError();
+    BoxStream bs{};  // one block on its own: no carry, the last window reported
+    bs.carry = x;
+    return launch_boxcar_stream(x, ndms, n, stride, bs, wmask, thr, sqrtw, cands, cap, count,
+                                mean, sigma, true, s);
 }
 
 // ---------------------------------------------------------------------------

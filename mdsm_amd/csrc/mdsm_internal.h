@@ -231,6 +231,7 @@ struct TimedLaunch {
 // Launchers (kernels.hip).
 hipError_t launch_copy2d(void *dst, int64_t dpitch, const void *src, int64_t spitch,
                          int64_t width, uint32_t height, hipStream_t s);
+hipError_t launch_zero2d(void *dst, int64_t dpitch, int64_t width, uint32_t height, hipStream_t s);
 hipError_t launch_plain(const float *in, int64_t in_stride, uint32_t nchans,
                         const int32_t *shift, uint32_t ndms, uint32_t nout, float *out,
                         hipStream_t s);
@@ -252,6 +253,27 @@ struct SqrtW {
     double v[11];  // sqrt(2^i), i = 0..10, computed on the host
 };
 hipError_t launch_fill_u32(uint32_t *p, uint32_t n, uint32_t v, hipStream_t s);
+// Streaming boxcar search: the searched series of row d is H carried samples
+// carry[d*carry_stride + 0..H) followed by the block's n samples; windows
+// start at t >= H - w (t >= 0), and tail = 1 leaves the window without a
+// right neighbour to the next block (H = 0, tail = 0: one block on its own).
+// Candidate t is relative to the block's first sample.
+struct BoxStream {
+    const float *carry;
+    int64_t carry_stride;
+    int32_t H;
+    int32_t tail;
+};
+uint32_t boxcar_maxw(uint32_t wmask);
+// Carried samples per row a stream with these widths needs: maxw + 1, rounded
+// up to a multiple of 4 (so that 16-byte loads never straddle the two sources).
+uint32_t boxcar_carry_len(uint32_t wmask);
+// stats = false: mean/sigma already hold the row statistics of the block.
+hipError_t launch_boxcar_stream(const float *x, uint32_t ndms, uint32_t n, int64_t stride,
+                                const BoxStream &bs, uint32_t wmask, double thr,
+                                const SqrtW &sqrtw, mdsm_cand *cands, uint32_t cap,
+                                uint32_t *count, double *mean, double *sigma, bool stats,
+                                hipStream_t s);
 hipError_t launch_boxcar(const float *x, uint32_t ndms, uint32_t n, int64_t stride,
                          uint32_t wmask, double thr, const SqrtW &sqrtw, mdsm_cand *cands,
                          uint32_t cap, uint32_t *count, double *mean, double *sigma,

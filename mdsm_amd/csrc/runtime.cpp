@@ -86,10 +86,21 @@ struct DevPool {
     }
 };
 
+// Kinds of queued blocks.
+enum : int32_t {
+    PK_HOST = 0,    // samples in a pinned staging block
+    PK_DEVICE = 1,  // samples in the caller's device buffer (mdsm_beam_push_device)
+    PK_ZERO = 2,    // a missing block, processed as zeros
+    PK_SKIP = 3     // a missing block whose history is all zeros: output known to be zero
+};
+
 struct Pending {
     uint64_t seq;
     double t_arrival;
-    void *staging;
+    void *staging;        // PK_HOST
+    const void *dsrc;     // PK_DEVICE
+    int64_t dstride;      // PK_DEVICE, elements
+    int32_t kind;
 };
 
 struct Beam;
@@ -98,6 +109,11 @@ struct Job {
     Beam *beam = nullptr;
     uint64_t seq = 0;
     double t_arrival = 0, t_admit = 0;
+    int32_t kind = PK_HOST;
+    const void *dsrc = nullptr;
+    int64_t dstride = 0;
+    bool carry_valid = false;  // the beam's carried samples continue this block's series
+    bool carry_zero = false;   // ... and are zeros not yet written (after skipped blocks)
     void *staging = nullptr;
     void *buf = nullptr;     // transient device allocation
     uint64_t held = 0;       // its size class
@@ -120,6 +136,15 @@ struct Beam {
     mdsm_detect_config det{};
     uint32_t wmask = 0;
     void *ring = nullptr, *tail = nullptr;
+    float *carry = nullptr;    // detector carry [ndms_total][H]
+    uint32_t H = 0;            // carried samples per DM row (0: no detection)
+    bool carry_valid = false;  // set once a block has filled the carry
+    bool carry_zero = false;   // carry is zeros not yet written (skipped blocks)
+    uint32_t zero_run = 0;     // consecutive zero-filled blocks at the end of the queue
+    uint64_t nfill = 0;        // zero-filled entries in the queue (not counted against its depth)
+    uint32_t zneed = 1;        // zero blocks after which the history is all zeros
+    int pushing = 0;           // pushes copying into staging without the lock
+    bool draining = false;     // admitted while the server is held (close, export)
     uint64_t resident = 0;
     size_t esize = 1;
     std::deque<Pending> queue;
@@ -128,6 +153,7 @@ struct Beam {
     std::vector<void *> acquired;   // handed out by mdsm_beam_acquire, not yet committed
     bool busy = false;
     bool closing = false;
+    bool cancelling = false;
     uint64_t next_seq = 0;
     uint64_t trans_bytes = 0;   // transient bytes per block
     uint64_t off_out = 0, off_work = 0, off_cands = 0, off_count = 0, off_mean = 0,
@@ -152,6 +178,18 @@ struct Done {
     mdsm_result res;
     std::vector<mdsm_cand> cands;
 };
+
+Done host_result(Beam *b, const Pending &p, int status, uint32_t flags)
+{
+    Done d{};
+    d.res.beam = b->id;
+    d.res.seq = p.seq;
+    d.res.status = status;
+    d.res.flags = flags;
+    d.res.t_arrival = p.t_arrival;
+    d.res.t_admit = d.res.t_done = mdsm_now();
+    return d;
+}
 
 }  // namespace
 
@@ -215,11 +253,13 @@ void finish_job(mdsm_server *s, Job *j)
     d.res.batch = j->batch;
     d.res.t_arrival = j->t_arrival;
     d.res.t_admit = j->t_admit;
+    if (j->kind == PK_ZERO) d.res.flags |= MDSM_RESULT_ZERO_FILLED;
     uint32_t total = 0;
     if (j->status == MDSM_OK && b->wmask) {
         for (uint32_t r = 0; r < nr; ++r) {
             const uint32_t n = b->h_count[r];
             total += n;
+            if (n > cap) d.res.flags |= MDSM_RESULT_OVERFLOW;
             const uint32_t first = b->plan->ranges[r].info.first_dm;
             for (uint32_t i = 0; i < std::min(n, cap); ++i) {
                 mdsm_cand c = b->h_cands[(size_t)r * cap + i];
@@ -234,13 +274,14 @@ void finish_job(mdsm_server *s, Job *j)
         });
     }
     d.res.ncand = total;
-    if (j->status == MDSM_OK && b->det.clip_nsigma > 0.0f && b->obs.dtype == MDSM_U8) {
+    if (j->status == MDSM_OK && b->det.clip_nsigma > 0.0f && b->obs.dtype == MDSM_U8 &&
+        j->kind != PK_ZERO) {
         uint64_t ncl = 0;
         for (uint32_t c = 0; c < b->obs.nchans; ++c) ncl += (uint32_t)b->h_ncl[c];
         d.res.nclipped = (uint32_t)std::min<uint64_t>(ncl, 0xffffffffu);
     }
     d.res.t_done = mdsm_now();
-    b->free_staging.push_back(j->staging);
+    if (j->staging) b->free_staging.push_back(j->staging);
     b->busy = false;
     if (b->det.retain_output && j->status == MDSM_OK) {
         b->retained[j->seq] = j;
@@ -262,19 +303,33 @@ int launch_batch(mdsm_server *s, Batch *bt)
         Beam *b = j->beam;
         const size_t C = b->obs.nchans, M = b->M, N = b->N, es = b->esize;
         const size_t S = M + N;
-        // one contiguous DMA of the block, then a kernel places it behind the
-        // overlap (the clipper when it runs, a copy otherwise)
-        hipError_t e = hipMemcpyAsync(j->dstage, j->staging, C * N * es, hipMemcpyHostToDevice, st);
-        if (e != hipSuccess) return hip_fail(e, "server H2D");
-        if (b->det.clip_nsigma > 0.0f && b->obs.dtype == MDSM_U8) {
+        // one contiguous DMA of the block (or the caller's device block, or
+        // zeros), then a kernel places it behind the overlap (the clipper
+        // when it runs, a copy otherwise)
+        hipError_t e = hipSuccess;
+        const void *src = j->dstage;
+        int64_t sstride = (int64_t)N;
+        if (j->kind == PK_HOST) {
+            e = hipMemcpyAsync(j->dstage, j->staging, C * N * es, hipMemcpyHostToDevice, st);
+            if (e != hipSuccess) return hip_fail(e, "server H2D");
+        } else if (j->kind == PK_DEVICE) {
+            src = j->dsrc;
+            sstride = j->dstride;
+        }
+        if (j->kind == PK_ZERO) {
+            // a zero block is unchanged by the clipper (median 0, MAD 0)
+            e = launch_zero2d((char *)b->ring + M * es, (int64_t)(S * es), (int64_t)(N * es),
+                              (uint32_t)C, st);
+            if (e != hipSuccess) return hip_fail(e, "server zero fill");
+        } else if (b->det.clip_nsigma > 0.0f && b->obs.dtype == MDSM_U8) {
             TimedLaunch tl("clip", st);
-            e = launch_clip_u8((const uint8_t *)j->dstage, (int64_t)N, (uint32_t)C, 0, (uint32_t)N,
+            e = launch_clip_u8((const uint8_t *)src, sstride, (uint32_t)C, 0, (uint32_t)N,
                                b->det.clip_nsigma, (uint8_t *)b->ring + M, (int64_t)S, nullptr,
                                nullptr, j->ncl, st);
             if (e != hipSuccess) return hip_fail(e, "server clip");
         } else {
-            e = launch_copy2d((char *)b->ring + M * es, (int64_t)(S * es), j->dstage,
-                              (int64_t)(N * es), (int64_t)(N * es), (uint32_t)C, st);
+            e = launch_copy2d((char *)b->ring + M * es, (int64_t)(S * es), src,
+                              (int64_t)(sstride * es), (int64_t)(N * es), (uint32_t)C, st);
             if (e != hipSuccess) return hip_fail(e, "server block placement");
         }
         if (!b->plan->decim.empty()) {
@@ -374,14 +429,32 @@ int launch_batch(mdsm_server *s, Batch *bt)
         if (b->wmask) {
             SqrtW sw;
             for (int i = 0; i < 11; ++i) sw.v[i] = std::sqrt((double)(1u << i));
+            const uint32_t H = b->H;
+            if (j->carry_zero) {  // skipped blocks left a zero history
+                e = launch_zero2d(b->carry, (int64_t)H * 4, (int64_t)H * 4,
+                                  b->plan->info.ndms_total, st);
+                if (e != hipSuccess) return hip_fail(e, "server carry reset");
+            }
             for (uint32_t r = 0; r < nr; ++r) {
                 const mdsm_range_info &ri = b->plan->ranges[r].info;
-                const uint32_t n = std::min<uint32_t>(b->N / ri.downsample, ri.nout);
-                TimedLaunch tl("detect", st);
-                e = launch_boxcar(j->out + ri.out_offset, ri.ndms, n, ri.nout, b->wmask,
-                                  b->det.snr_thr, sw, j->cands + (size_t)r * cap, cap,
-                                  j->count + r, j->mean + ri.first_dm, j->sigma + ri.first_dm, st);
+                // the block's share of the range's series (the rest of each row
+                // repeats the next block's first samples)
+                const uint32_t n = b->N / ri.downsample;
+                float *carry = b->carry + (size_t)ri.first_dm * H;
+                const BoxStream bs{carry, (int64_t)H, j->carry_valid ? (int32_t)H : 0, 1};
+                {
+                    TimedLaunch tl("detect", st);
+                    e = launch_boxcar_stream(j->out + ri.out_offset, ri.ndms, n, ri.nout, bs,
+                                             b->wmask, b->det.snr_thr, sw,
+                                             j->cands + (size_t)r * cap, cap, j->count + r,
+                                             j->mean + ri.first_dm, j->sigma + ri.first_dm, true,
+                                             st);
+                }
                 if (e != hipSuccess) return hip_fail(e, "server detect");
+                // the last H samples of each row carry over to the next block
+                e = launch_copy2d(carry, (int64_t)H * 4, j->out + ri.out_offset + (n - H),
+                                  (int64_t)ri.nout * 4, (int64_t)H * 4, ri.ndms, st);
+                if (e != hipSuccess) return hip_fail(e, "server carry save");
             }
             e = hipMemcpyAsync(b->h_count, j->count, nr * 4, hipMemcpyDeviceToHost, st);
             if (e == hipSuccess)
@@ -389,7 +462,7 @@ int launch_batch(mdsm_server *s, Batch *bt)
                                    hipMemcpyDeviceToHost, st);
             if (e != hipSuccess) return hip_fail(e, "server D2H");
         }
-        if (b->det.clip_nsigma > 0.0f && b->obs.dtype == MDSM_U8) {
+        if (b->det.clip_nsigma > 0.0f && b->obs.dtype == MDSM_U8 && j->kind != PK_ZERO) {
             e = hipMemcpyAsync(b->h_ncl, j->ncl, C * 4, hipMemcpyDeviceToHost, st);
             if (e != hipSuccess) return hip_fail(e, "server D2H");
         }
@@ -424,15 +497,35 @@ void scheduler(mdsm_server *s)
             progress = true;
         }
         if (progress) s->cv_done.notify_all();
+        // blocks whose output is known to be zero complete on the host, in
+        // their beam's order
+        for (auto &kv : s->beams) {
+            Beam *b = kv.second;
+            while (!b->busy && !b->queue.empty() && b->queue.front().kind == PK_SKIP) {
+                const Pending pd = b->queue.front();
+                b->queue.pop_front();
+                b->nfill--;
+                s->done.push_back(
+                    host_result(b, pd, MDSM_OK, MDSM_RESULT_ZERO_FILLED | MDSM_RESULT_SKIPPED));
+                s->st.completed++;
+                if (b->wmask) {
+                    b->carry_zero = true;
+                    b->carry_valid = true;
+                }
+                progress = true;
+            }
+        }
+        if (progress) s->cv_done.notify_all();
         // admit a batch
         Batch *bt = nullptr;
-        if (!s->held && s->inflight.size() < (size_t)kStreams * 2) {
+        if (s->inflight.size() < (size_t)kStreams * 2) {
             std::vector<Beam *> order;
             for (auto &kv : s->beams) order.push_back(kv.second);
             const size_t nb = order.size();
             for (size_t i = 0; i < nb; ++i) {
                 Beam *b = order[(s->rr + i) % nb];
-                if (b->busy || b->queue.empty()) continue;
+                if (b->busy || b->queue.empty() || b->queue.front().kind == PK_SKIP) continue;
+                if (s->held && !b->draining) continue;
                 if (!bt) {
                     bt = new Batch();
                     bt->stream = s->next_stream;
@@ -450,12 +543,20 @@ void scheduler(mdsm_server *s)
                 }
                 Pending pd = b->queue.front();
                 b->queue.pop_front();
+                if (pd.kind == PK_ZERO) b->nfill--;
                 Job *j = new Job();
                 j->beam = b;
                 j->seq = pd.seq;
                 j->t_arrival = pd.t_arrival;
                 j->t_admit = mdsm_now();
                 j->staging = pd.staging;
+                j->kind = pd.kind;
+                j->dsrc = pd.dsrc;
+                j->dstride = pd.dstride;
+                j->carry_valid = b->carry_valid;
+                j->carry_zero = b->carry_zero;
+                b->carry_valid = b->wmask != 0;
+                b->carry_zero = false;
                 j->buf = buf;
                 j->held = held;
                 char *base = (char *)buf;
@@ -530,7 +631,8 @@ int upload_and_alloc_beam(mdsm_server *s, Beam *b)
     uint64_t tables = b->plan->blocks.size() * sizeof(BlockDesc) +
                       b->plan->rel.size() * 4 + b->plan->abase.size() * 4 +
                       b->plan->segs.size() * sizeof(SegDesc);
-    b->resident = ring_bytes + tail_bytes + up256(tables);
+    const uint64_t carry_bytes = up256((uint64_t)b->plan->info.ndms_total * b->H * 4);
+    b->resident = ring_bytes + tail_bytes + carry_bytes + up256(tables);
     if (s->pool.used + b->resident + up256(b->trans_bytes) > s->pool.budget) {
         s->pool.clear();
         if (s->pool.used + b->resident + up256(b->trans_bytes) > s->pool.budget)
@@ -547,6 +649,10 @@ int upload_and_alloc_beam(mdsm_server *s, Beam *b)
     if (tail_bytes) {
         e = hipMalloc(&b->tail, tail_bytes);
         if (e != hipSuccess) return hip_fail(e, "beam tail");
+    }
+    if (b->H) {
+        e = hipMalloc((void **)&b->carry, carry_bytes);
+        if (e != hipSuccess) return hip_fail(e, "beam detector carry");
     }
     s->pool.used += b->resident;
     s->pool.peak = std::max(s->pool.peak, s->pool.used);
@@ -579,6 +685,7 @@ void free_beam(mdsm_server *s, Beam *b)
         s->pool.used -= b->resident;
     }
     if (b->tail) (void)hipFree(b->tail);
+    if (b->carry) (void)hipFree(b->carry);
     for (void *p : b->all_staging) (void)hipHostFree(p);
     if (b->h_cands) (void)hipHostFree(b->h_cands);
     if (b->h_count) (void)hipHostFree(b->h_count);
@@ -696,11 +803,23 @@ extern "C" int mdsm_beam_open(mdsm_server *s, const mdsm_obs *obs, const mdsm_ra
         return fail(MDSM_ERR_INVALID, "block length N must be a multiple of every downsample");
     }
     b->M = M;
+    b->zneed = M ? (M + b->N - 1) / b->N : 1;
     o.nsamp_in = M + b->N;
     rc = mdsm_plan_create(&o, ranges, nranges, &b->plan);
     if (rc) {
         delete b;
         return rc;
+    }
+    if (b->wmask) {
+        b->H = boxcar_carry_len(b->wmask);
+        for (auto &rh : b->plan->ranges)
+            if (b->N / rh.info.downsample < b->H) {
+                mdsm_plan_destroy(b->plan);
+                delete b;
+                return fail(MDSM_ERR_INVALID,
+                            "block too short for the detector: N/downsample must be >= " +
+                                std::to_string(boxcar_carry_len(b->wmask)));
+            }
     }
     beam_transient_layout(b, b->det.cand_cap);
     std::lock_guard<std::mutex> g(s->mu);
@@ -741,10 +860,11 @@ extern "C" int mdsm_beam_push(mdsm_server *s, uint32_t id, const void *host, int
         b = it->second;
         if (b->closing) return fail(MDSM_ERR_STATE, "beam is closing");
         if (stride < (int64_t)b->N) return fail(MDSM_ERR_INVALID, "stride < block length");
-        if (b->queue.size() >= s->cfg.queue_depth || b->free_staging.empty())
+        if (b->queue.size() - b->nfill >= s->cfg.queue_depth || b->free_staging.empty())
             return fail(MDSM_ERR_AGAIN, "beam queue full");
         stg = b->free_staging.back();
         b->free_staging.pop_back();
+        b->pushing++;  // close and cancel wait for the copy below
     }
     const size_t C = b->obs.nchans, N = b->N, es = b->esize;
     auto copy_rows = [&](size_t c0, size_t c1) {
@@ -764,8 +884,77 @@ extern "C" int mdsm_beam_push(mdsm_server *s, uint32_t id, const void *host, int
     }
     {
         std::lock_guard<std::mutex> g(s->mu);
-        Pending p{b->next_seq++, arrival_s >= 0 ? arrival_s : mdsm_now(), stg};
+        b->pushing--;
+        if (b->cancelling) {
+            b->free_staging.push_back(stg);
+            s->cv_done.notify_all();
+            return fail(MDSM_ERR_STATE, "beam was cancelled");
+        }
+        Pending p{b->next_seq++, arrival_s >= 0 ? arrival_s : mdsm_now(), stg, nullptr, 0,
+                  PK_HOST};
         b->queue.push_back(p);
+        b->zero_run = 0;
+        s->st.pushed++;
+        if (seq) *seq = p.seq;
+    }
+    s->cv_work.notify_all();
+    s->cv_done.notify_all();
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_beam_push_seq(mdsm_server *s, uint32_t id, const void *host, int64_t stride,
+                                  uint64_t seq, double arrival_s)
+{
+    if (!s || !host) return fail(MDSM_ERR_INVALID, "NULL argument");
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->beams.find(id);
+        if (it == s->beams.end()) return fail(MDSM_ERR_INVALID, "no such beam");
+        Beam *b = it->second;
+        if (b->closing) return fail(MDSM_ERR_STATE, "beam is closing");
+        if (seq < b->next_seq)
+            return fail(MDSM_ERR_STATE, "block " + std::to_string(seq) +
+                                            " is a duplicate or late (next expected " +
+                                            std::to_string(b->next_seq) + ")");
+        const uint64_t gap = seq - b->next_seq;
+        if (gap > (1u << 20)) return fail(MDSM_ERR_INVALID, "sequence gap longer than 2^20 blocks");
+        if (stride < (int64_t)b->N) return fail(MDSM_ERR_INVALID, "stride < block length");
+        if (b->queue.size() - b->nfill >= s->cfg.queue_depth || b->free_staging.empty())
+            return fail(MDSM_ERR_AGAIN, "beam queue full");
+        // the missing blocks go first, as zeros; once M samples of zeros are
+        // in the history a zero block's output is exactly zero
+        const double now = arrival_s >= 0 ? arrival_s : mdsm_now();
+        for (uint64_t i = 0; i < gap; ++i) {
+            const int32_t kind = b->zero_run >= b->zneed ? PK_SKIP : PK_ZERO;
+            b->queue.push_back(Pending{b->next_seq++, now, nullptr, nullptr, 0, kind});
+            b->zero_run++;
+            b->nfill++;
+            s->st.pushed++;
+        }
+    }
+    uint64_t got = 0;
+    int rc = mdsm_beam_push(s, id, host, stride, arrival_s, &got);
+    if (rc == MDSM_OK && got != seq) return fail(MDSM_ERR_STATE, "concurrent pushes to one beam");
+    return rc;
+}
+
+extern "C" int mdsm_beam_push_device(mdsm_server *s, uint32_t id, const void *d_block,
+                                     int64_t stride, double arrival_s, uint64_t *seq)
+{
+    if (!s || !d_block) return fail(MDSM_ERR_INVALID, "NULL argument");
+    {
+        std::lock_guard<std::mutex> g(s->mu);
+        auto it = s->beams.find(id);
+        if (it == s->beams.end()) return fail(MDSM_ERR_INVALID, "no such beam");
+        Beam *b = it->second;
+        if (b->closing) return fail(MDSM_ERR_STATE, "beam is closing");
+        if (stride < (int64_t)b->N) return fail(MDSM_ERR_INVALID, "stride < block length");
+        if (b->queue.size() - b->nfill >= s->cfg.queue_depth)
+            return fail(MDSM_ERR_AGAIN, "beam queue full");
+        Pending p{b->next_seq++, arrival_s >= 0 ? arrival_s : mdsm_now(), nullptr, d_block,
+                  stride, PK_DEVICE};
+        b->queue.push_back(p);
+        b->zero_run = 0;
         s->st.pushed++;
         if (seq) *seq = p.seq;
     }
@@ -789,8 +978,9 @@ extern "C" int mdsm_server_poll(mdsm_server *s, mdsm_result *res, mdsm_cand *can
     Done d = std::move(s->done.front());
     s->done.pop_front();
     *res = d.res;
-    if (cands && cap)
-        std::memcpy(cands, d.cands.data(), std::min<size_t>(cap, d.cands.size()) * sizeof(mdsm_cand));
+    const size_t nret = (cands && cap) ? std::min<size_t>(cap, d.cands.size()) : 0;
+    if (nret) std::memcpy(cands, d.cands.data(), nret * sizeof(mdsm_cand));
+    res->nreturned = (uint32_t)nret;
     return MDSM_OK;
 }
 
@@ -848,21 +1038,184 @@ extern "C" int mdsm_server_drain(mdsm_server *s)
     }
 }
 
-extern "C" int mdsm_beam_close(mdsm_server *s, uint32_t id)
+namespace {
+
+// Wait (lock held) until the beam has nothing queued, on the device or being
+// copied in.  Its blocks are admitted even while the server is held.
+void wait_beam_idle(mdsm_server *s, Beam *b, std::unique_lock<std::mutex> &lk)
 {
-    if (!s) return fail(MDSM_ERR_INVALID, "NULL argument");
+    b->draining = true;
+    while (!b->queue.empty() || b->busy || b->pushing > 0) {
+        s->cv_work.notify_all();
+        s->cv_done.wait_for(lk, std::chrono::milliseconds(5));
+    }
+    b->draining = false;
+}
+
+int close_beam(mdsm_server *s, uint32_t id, bool cancel)
+{
     std::unique_lock<std::mutex> lk(s->mu);
     auto it = s->beams.find(id);
     if (it == s->beams.end()) return fail(MDSM_ERR_INVALID, "no such beam");
     Beam *b = it->second;
+    if (b->closing) return fail(MDSM_ERR_STATE, "beam is already closing");
     b->closing = true;
-    while (!b->queue.empty() || b->busy) {
-        s->cv_work.notify_all();
-        s->cv_done.wait_for(lk, std::chrono::milliseconds(5));
+    if (cancel) {
+        b->cancelling = true;
+        for (;;) {
+            // drop what is queued (pushes still copying drop their own block)
+            bool dropped = false;
+            while (!b->queue.empty()) {
+                const Pending pd = b->queue.front();
+                b->queue.pop_front();
+                if (pd.kind == PK_ZERO || pd.kind == PK_SKIP) b->nfill--;
+                if (pd.staging) b->free_staging.push_back(pd.staging);
+                s->done.push_back(host_result(b, pd, MDSM_ERR_CANCELLED, 0));
+                dropped = true;
+            }
+            if (dropped) s->cv_done.notify_all();
+            if (!b->busy && b->pushing == 0) break;
+            s->cv_done.wait_for(lk, std::chrono::milliseconds(5));
+        }
+    } else {
+        wait_beam_idle(s, b, lk);
     }
     s->beams.erase(it);
     s->st.beams = (uint32_t)s->beams.size();
     free_beam(s, b);
+    return MDSM_OK;
+}
+
+}  // namespace
+
+extern "C" int mdsm_beam_close(mdsm_server *s, uint32_t id)
+{
+    if (!s) return fail(MDSM_ERR_INVALID, "NULL argument");
+    return close_beam(s, id, false);
+}
+
+extern "C" int mdsm_beam_cancel(mdsm_server *s, uint32_t id)
+{
+    if (!s) return fail(MDSM_ERR_INVALID, "NULL argument");
+    return close_beam(s, id, true);
+}
+
+namespace {
+
+// Serialised beam state (mdsm_beam_export): this header, the overlap
+// [nchans][M] samples of the beam's dtype, then the carry [ndms][H] floats.
+struct BeamStateHeader {
+    char magic[8];
+    uint32_t nchans, N, M, esize, ndms, H, carry_valid, zero_run;
+    uint64_t next_seq;
+    uint64_t shape;  // hash of the plan's ranges, widths and clip setting
+};
+constexpr char kStateMagic[8] = {'M', 'D', 'S', 'M', 'B', 'S', 'T', '1'};
+
+uint64_t beam_shape_hash(const Beam *b)
+{
+    uint64_t h = 1469598103934665603ull;
+    auto mix = [&](const void *p, size_t n) {
+        for (size_t i = 0; i < n; ++i) {
+            h ^= ((const unsigned char *)p)[i];
+            h *= 1099511628211ull;
+        }
+    };
+    mix(&b->obs, sizeof(b->obs));
+    for (const auto &rh : b->plan->ranges) {
+        mix(&rh.info.lowdm, sizeof(float));
+        mix(&rh.info.dmstep, sizeof(float));
+        mix(&rh.info.ndms, sizeof(uint32_t));
+        mix(&rh.info.downsample, sizeof(uint32_t));
+    }
+    mix(&b->wmask, sizeof(b->wmask));
+    mix(&b->det.clip_nsigma, sizeof(float));
+    return h;
+}
+
+uint64_t beam_state_bytes(const Beam *b)
+{
+    return sizeof(BeamStateHeader) + (uint64_t)b->obs.nchans * b->M * b->esize +
+           (uint64_t)b->plan->info.ndms_total * b->H * 4;
+}
+
+}  // namespace
+
+extern "C" int mdsm_beam_export(mdsm_server *s, uint32_t id, void *host, uint64_t cap,
+                                uint64_t *bytes)
+{
+    if (!s || !bytes) return fail(MDSM_ERR_INVALID, "NULL argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->beams.find(id);
+    if (it == s->beams.end()) return fail(MDSM_ERR_INVALID, "no such beam");
+    Beam *b = it->second;
+    *bytes = beam_state_bytes(b);
+    if (!host || cap < *bytes) return fail(MDSM_ERR_CAPACITY, "state buffer too small");
+    if (b->closing) return fail(MDSM_ERR_STATE, "beam is closing");
+    b->closing = true;  // no pushes while the state is read
+    wait_beam_idle(s, b, lk);
+    BeamStateHeader h{};
+    std::memcpy(h.magic, kStateMagic, 8);
+    h.nchans = b->obs.nchans;
+    h.N = b->N;
+    h.M = b->M;
+    h.esize = (uint32_t)b->esize;
+    h.ndms = b->plan->info.ndms_total;
+    h.H = b->H;
+    h.carry_valid = b->carry_valid ? 1u : 0u;
+    h.zero_run = b->zero_run;
+    h.next_seq = b->next_seq;
+    h.shape = beam_shape_hash(b);
+    char *p = (char *)host;
+    std::memcpy(p, &h, sizeof(h));
+    p += sizeof(h);
+    (void)hipSetDevice(s->device);
+    const size_t C = b->obs.nchans, S = (size_t)b->M + b->N, es = b->esize;
+    hipError_t e = hipSuccess;
+    if (b->M)
+        e = hipMemcpy2D(p, b->M * es, b->ring, S * es, b->M * es, C, hipMemcpyDeviceToHost);
+    p += C * b->M * es;
+    const size_t cbytes = (size_t)h.ndms * b->H * 4;
+    if (e == hipSuccess && cbytes) {
+        if (b->carry_zero)
+            std::memset(p, 0, cbytes);
+        else
+            e = hipMemcpy(p, b->carry, cbytes, hipMemcpyDeviceToHost);
+    }
+    b->closing = false;
+    if (e != hipSuccess) return hip_fail(e, "beam export");
+    return MDSM_OK;
+}
+
+extern "C" int mdsm_beam_import(mdsm_server *s, uint32_t id, const void *host, uint64_t bytes)
+{
+    if (!s || !host) return fail(MDSM_ERR_INVALID, "NULL argument");
+    std::unique_lock<std::mutex> lk(s->mu);
+    auto it = s->beams.find(id);
+    if (it == s->beams.end()) return fail(MDSM_ERR_INVALID, "no such beam");
+    Beam *b = it->second;
+    if (b->next_seq != 0 || b->busy || !b->queue.empty() || b->pushing || b->closing)
+        return fail(MDSM_ERR_STATE, "import needs a beam no block was pushed to");
+    if (bytes != beam_state_bytes(b)) return fail(MDSM_ERR_INVALID, "state size does not match");
+    BeamStateHeader h;
+    std::memcpy(&h, host, sizeof(h));
+    if (std::memcmp(h.magic, kStateMagic, 8) != 0) return fail(MDSM_ERR_INVALID, "not a beam state");
+    if (h.nchans != b->obs.nchans || h.N != b->N || h.M != b->M || h.esize != b->esize ||
+        h.ndms != b->plan->info.ndms_total || h.H != b->H || h.shape != beam_shape_hash(b))
+        return fail(MDSM_ERR_INVALID, "beam state is of another observation, plan or detector");
+    const char *p = (const char *)host + sizeof(h);
+    (void)hipSetDevice(s->device);
+    const size_t C = b->obs.nchans, S = (size_t)b->M + b->N, es = b->esize;
+    hipError_t e = hipSuccess;
+    if (b->M) e = hipMemcpy2D(b->ring, S * es, p, b->M * es, b->M * es, C, hipMemcpyHostToDevice);
+    p += C * b->M * es;
+    const size_t cbytes = (size_t)h.ndms * b->H * 4;
+    if (e == hipSuccess && cbytes) e = hipMemcpy(b->carry, p, cbytes, hipMemcpyHostToDevice);
+    if (e != hipSuccess) return hip_fail(e, "beam import");
+    b->next_seq = h.next_seq;
+    b->carry_valid = h.carry_valid != 0;
+    b->carry_zero = false;
+    b->zero_run = h.zero_run;
     return MDSM_OK;
 }
 
@@ -885,7 +1238,7 @@ extern "C" int mdsm_beam_acquire(mdsm_server *s, uint32_t id, void **host_block)
     if (it == s->beams.end()) return fail(MDSM_ERR_INVALID, "no such beam");
     Beam *b = it->second;
     if (b->closing) return fail(MDSM_ERR_STATE, "beam is closing");
-    if (b->queue.size() + b->acquired.size() >= s->cfg.queue_depth || b->free_staging.empty())
+    if (b->queue.size() - b->nfill + b->acquired.size() >= s->cfg.queue_depth || b->free_staging.empty())
         return fail(MDSM_ERR_AGAIN, "beam queue full");
     void *stg = b->free_staging.back();
     b->free_staging.pop_back();
@@ -911,8 +1264,10 @@ extern "C" int mdsm_beam_commit(mdsm_server *s, uint32_t id, void *host_block, d
             b->free_staging.push_back(host_block);
             return fail(MDSM_ERR_STATE, "beam is closing");
         }
-        Pending p{b->next_seq++, arrival_s >= 0 ? arrival_s : mdsm_now(), host_block};
+        Pending p{b->next_seq++, arrival_s >= 0 ? arrival_s : mdsm_now(), host_block, nullptr, 0,
+                  PK_HOST};
         b->queue.push_back(p);
+        b->zero_run = 0;
         s->st.pushed++;
         if (seq) *seq = p.seq;
     }

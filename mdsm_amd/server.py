@@ -30,9 +30,14 @@ class DetectConfig(C.Structure):
 
 class Result(C.Structure):
     _fields_ = [("beam", C.c_uint32), ("status", C.c_int32), ("seq", C.c_uint64),
-                ("ncand", C.c_uint32), ("nclipped", C.c_uint32), ("batch", C.c_uint32),
-                ("pad", C.c_uint32), ("t_arrival", C.c_double), ("t_admit", C.c_double),
-                ("t_done", C.c_double)]
+                ("ncand", C.c_uint32), ("nreturned", C.c_uint32), ("nclipped", C.c_uint32),
+                ("batch", C.c_uint32), ("flags", C.c_uint32), ("pad", C.c_uint32),
+                ("t_arrival", C.c_double), ("t_admit", C.c_double), ("t_done", C.c_double)]
+
+
+# Result.flags
+OVERFLOW, ZERO_FILLED, SKIPPED = 1, 2, 4
+CANCELLED = -9
 
 
 class ServerStats(C.Structure):
@@ -57,6 +62,12 @@ def _sig():
         "mdsm_beam_open": [vp, C.POINTER(Obs), C.POINTER(Range), C.c_uint32,
                            C.POINTER(DetectConfig), C.POINTER(C.c_uint32)],
         "mdsm_beam_close": [vp, C.c_uint32],
+        "mdsm_beam_cancel": [vp, C.c_uint32],
+        "mdsm_beam_push_seq": [vp, C.c_uint32, vp, C.c_int64, C.c_uint64, C.c_double],
+        "mdsm_beam_push_device": [vp, C.c_uint32, vp, C.c_int64, C.c_double,
+                                  C.POINTER(C.c_uint64)],
+        "mdsm_beam_export": [vp, C.c_uint32, vp, C.c_uint64, C.POINTER(C.c_uint64)],
+        "mdsm_beam_import": [vp, C.c_uint32, vp, C.c_uint64],
         "mdsm_beam_info": [vp, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(PlanInfo)],
         "mdsm_beam_push": [vp, C.c_uint32, vp, C.c_int64, C.c_double, C.POINTER(C.c_uint64)],
         "mdsm_beam_acquire": [vp, C.c_uint32, C.POINTER(vp)],
@@ -122,8 +133,50 @@ class Server:
         return b
 
     def close_beam(self, b: Beam):
+        """Process every block already pushed, then free the beam."""
         check(_sig().mdsm_beam_close(self.h, b.id))
         self.beams.pop(b.id, None)
+
+    def cancel_beam(self, b: Beam):
+        """Drop the beam's queued blocks (reported with status CANCELLED) and
+        free it once its block on the device has completed."""
+        check(_sig().mdsm_beam_cancel(self.h, b.id))
+        self.beams.pop(b.id, None)
+
+    def push_seq(self, b: Beam, block: np.ndarray, seq: int, arrival=-1.0):
+        """Push block number `seq`: gaps are zero-filled, duplicates raise
+        (MdsmError -6); returns False on back-pressure."""
+        block = np.ascontiguousarray(block, NP_DTYPE[b.obs.dtype])
+        rc = _sig().mdsm_beam_push_seq(self.h, b.id, block.ctypes.data, block.shape[1], seq,
+                                       arrival)
+        if rc == -8:
+            return False
+        check(rc)
+        return True
+
+    def push_device(self, b: Beam, d_block, stride: int, arrival=-1.0):
+        """Queue a block already in device memory (kept unchanged by the
+        caller until its result is polled); None on back-pressure."""
+        seq = C.c_uint64()
+        ptr = d_block.ptr if hasattr(d_block, "ptr") else d_block
+        rc = _sig().mdsm_beam_push_device(self.h, b.id, ptr, stride, arrival, C.byref(seq))
+        if rc == -8:
+            return None
+        check(rc)
+        return seq.value
+
+    def export_state(self, b: Beam) -> bytes:
+        n = C.c_uint64()
+        rc = _sig().mdsm_beam_export(self.h, b.id, None, 0, C.byref(n))
+        if rc not in (0, -4):
+            check(rc)
+        buf = (C.c_char * n.value)()
+        check(_sig().mdsm_beam_export(self.h, b.id, buf, n.value, C.byref(n)))
+        return bytes(buf)
+
+    def import_state(self, b: Beam, state: bytes):
+        buf = (C.c_char * len(state)).from_buffer_copy(state)
+        check(_sig().mdsm_beam_import(self.h, b.id, buf, len(state)))
 
     def push(self, b: Beam, block: np.ndarray, arrival=-1.0) -> int:
         block = np.ascontiguousarray(block, NP_DTYPE[b.obs.dtype])
@@ -167,7 +220,7 @@ class Server:
         if rc == -8:
             return None, None
         check(rc)
-        n = min(res.ncand, cap)
+        n = res.nreturned
         arr = np.zeros(n, CAND_DTYPE)
         if n:
             raw = np.ctypeslib.as_array(buf)[:n]

@@ -26,8 +26,13 @@ _ref = None
 
 
 class Cand(C.Structure):
-    _fields_ = [("dm", C.c_uint32), ("t", C.c_uint32), ("width", C.c_uint32),
+    _fields_ = [("dm", C.c_uint32), ("t", C.c_int32), ("width", C.c_uint32),
                 ("snr", C.c_float)]
+
+
+class SCand(C.Structure):
+    _fields_ = [("dm", C.c_uint32), ("t", C.c_int32), ("width", C.c_uint32),
+                ("snr", C.c_float), ("blk", C.c_uint32)]
 
 
 def _p(a):
@@ -63,6 +68,8 @@ def lib():
         L.orc_row_stats.argtypes = [vp, i32, C.POINTER(f64), C.POINTER(f64)]
         L.orc_boxcar_detect.restype = i32
         L.orc_boxcar_detect.argtypes = [vp, i32, i32, i64, vp, i32, f64, vp, i32, vp, vp]
+        L.orc_boxcar_stream.restype = i32
+        L.orc_boxcar_stream.argtypes = [vp, i32, i32, i32, i64, vp, i32, f64, vp, i32]
         L.orc_median_of_five.restype = f32
         L.orc_median_of_five.argtypes = [f32] * 5
         L.orc_corner_turn_f32.restype = None
@@ -219,13 +226,40 @@ def boxcar_detect(x, widths, thr, cap=1 << 20):
         if cnt <= cap:
             break
         cap = cnt
-    arr = np.zeros(cnt, dtype=[("dm", np.uint32), ("t", np.uint32), ("width", np.uint32),
+    arr = np.zeros(cnt, dtype=[("dm", np.uint32), ("t", np.int32), ("width", np.uint32),
                                ("snr", np.float32)])
     if cnt:
         raw = np.ctypeslib.as_array(cands)[:cnt]
         for f in ("dm", "t", "width", "snr"):
             arr[f] = raw[f]
     return arr, means, sig
+
+
+def boxcar_stream(x, n, widths, thr, cap=1 << 20):
+    """Streaming boxcar search over rows x [ndms][nblk*n] taken as nblk
+    consecutive blocks of n samples (orc_boxcar_stream): per-block
+    statistics, each window searched once by the block its end falls in.
+    Returns candidates with fields dm, t (relative to the owning block's
+    first sample), width, snr, blk."""
+    x = np.ascontiguousarray(x, np.float32)
+    ndms, T = x.shape
+    if T % n:
+        raise ValueError("row length must be a whole number of blocks")
+    w = np.ascontiguousarray(widths, np.int32)
+    while True:
+        cands = (SCand * cap)()
+        cnt = lib().orc_boxcar_stream(_p(x), ndms, n, T // n, T, _p(w), w.size, thr,
+                                      C.cast(cands, C.c_void_p), cap)
+        if cnt <= cap:
+            break
+        cap = cnt
+    arr = np.zeros(cnt, dtype=[("dm", np.uint32), ("t", np.int32), ("width", np.uint32),
+                               ("snr", np.float32), ("blk", np.uint32)])
+    if cnt:
+        raw = np.ctypeslib.as_array(cands)[:cnt]
+        for f in ("dm", "t", "width", "snr", "blk"):
+            arr[f] = raw[f]
+    return arr
 
 
 def median_of_five(a, b, c, d, e):

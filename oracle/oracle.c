@@ -506,7 +506,7 @@ int orc_boxcar_detect(const float *x, int ndms, int n, int64_t stride,
                     continue;
                 if (count < cap) {
                     cands[count].dm = (uint32_t)d;
-                    cands[count].t = (uint32_t)t;
+                    cands[count].t = (int32_t)t;
                     cands[count].width = (uint32_t)w;
                     cands[count].snr = (float)snr;
                 }
@@ -518,6 +518,80 @@ int orc_boxcar_detect(const float *x, int ndms, int n, int64_t stride,
     free(nxt);
     free(byw);
     (void)maxw;
+    return count;
+}
+
+/* Streaming boxcar search over the concatenation X of nblk blocks of n
+ * samples per row.  Statistics are per (row, block): mean_k and sigma_k of
+ * X[k n, (k+1) n) as orc_row_stats forms them.  Window (t, w) ends at
+ * e = t + w - 1 and is owned by block k = (e + 1) / n, i.e. by the block
+ * holding e + 1, the last sample its right neighbour S_w(t+1) reads; it is
+ * searched when that neighbour exists (t + 1 + w <= nblk n), with block k's
+ * statistics:  snr = (S_w(t) - w mean_k) / (sigma_k sqrt(w)) > thr and
+ * (t == 0 or S_w(t) >= S_w(t-1)) and S_w(t) > S_w(t+1).  Boxcar sums are the
+ * pairwise tree of orc_boxcar_detect over X.  So every window is searched
+ * exactly once, the ones straddling a block edge included, and a stream
+ * searched block by block with the previous block's last samples carried
+ * over finds the same candidates.  No counterpart in the reference, whose
+ * detectors work block by block (src/mdsm/dedispersion_output.cpp:176-181;
+ * src/prototypes/singlepulse.cpp:41-146 convolves each 8192-sample chunk
+ * on its own). */
+int orc_boxcar_stream(const float *x, int ndms, int n, int nblk, int64_t stride,
+                      const int *widths, int nwidths, double thr, orc_scand *cands, int cap)
+{
+    const int64_t T = (int64_t)n * nblk;
+    double *cur = (double *)malloc((size_t)T * sizeof(double));
+    double *nxt = (double *)malloc((size_t)T * sizeof(double));
+    double *mean = (double *)malloc((size_t)nblk * sizeof(double));
+    double *sig = (double *)malloc((size_t)nblk * sizeof(double));
+    int count = 0;
+    for (int d = 0; d < ndms; d++) {
+        const float *row = x + (size_t)d * stride;
+        for (int k = 0; k < nblk; k++)
+            orc_row_stats(row + (int64_t)k * n, n, &mean[k], &sig[k]);
+        for (int wi = 0; wi < nwidths; wi++) {
+            const int w = widths[wi];
+            if ((int64_t)w + 1 > T)
+                continue;
+            for (int64_t t = 0; t < T; t++)
+                cur[t] = (double)row[t];
+            int cw = 1;
+            while (cw < w) {
+                for (int64_t t = 0; t + 2 * cw <= T; t++)
+                    nxt[t] = cur[t] + cur[t + cw];
+                double *tmp = cur;
+                cur = nxt;
+                nxt = tmp;
+                cw *= 2;
+            }
+            const double sw = sqrt((double)w);
+            for (int64_t t = 0; t + 1 + w <= T; t++) {
+                const int64_t k = (t + w) / n; /* block of e + 1 */
+                if (!(sig[k] > 0.0))
+                    continue;
+                const double s = cur[t];
+                const double snr = (s - (double)w * mean[k]) / (sig[k] * sw);
+                if (!(snr > thr))
+                    continue;
+                if (t > 0 && !(s >= cur[t - 1]))
+                    continue;
+                if (!(s > cur[t + 1]))
+                    continue;
+                if (count < cap) {
+                    cands[count].dm = (uint32_t)d;
+                    cands[count].t = (int32_t)(t - k * n);
+                    cands[count].width = (uint32_t)w;
+                    cands[count].snr = (float)snr;
+                    cands[count].blk = (uint32_t)k;
+                }
+                count++;
+            }
+        }
+    }
+    free(cur);
+    free(nxt);
+    free(mean);
+    free(sig);
     return count;
 }
 

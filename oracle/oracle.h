@@ -33,6 +33,11 @@
  *   orc_boxcar_detect
  *       closed forms (spike: snr_w = h/sqrt(w) pattern; top hat), brute
  *       force numpy on tiny inputs.
+ *   orc_boxcar_stream
+ *       with one block it is orc_boxcar_detect less the windows without a
+ *       right neighbour; numpy brute force over tiny multi-block series
+ *       (per-block statistics, window ownership by end sample); a pulse
+ *       straddling a block edge is found once, at its concatenated S/N.
  *   orc_median_of_five
  *       all 5^5 tuples against the reference's own medianOfFive built under
  *       oracle/_ref and against sort.
@@ -110,7 +115,7 @@ void orc_clip_u8(const uint8_t *in, int nchans, int64_t in_stride, int64_t t0,
 /* --- boxcar single-pulse detector ------------------------------------ */
 typedef struct {
     uint32_t dm;
-    uint32_t t;
+    int32_t t;
     uint32_t width;
     float snr;
 } orc_cand;
@@ -119,6 +124,21 @@ void orc_row_stats(const float *row, int n, double *mean, double *sigma);
 int orc_boxcar_detect(const float *x, int ndms, int n, int64_t stride,
                       const int *widths, int nwidths, double thr,
                       orc_cand *cands, int cap, double *means, double *sigmas);
+
+/* Streaming form over the concatenation of nblk blocks of n samples per row
+ * (row d: x[d*stride + 0 .. nblk*n)): every window is searched once, by the
+ * block its end falls in, with that block's statistics.  Candidate t is the
+ * window start relative to the first sample of that block (negative: the
+ * window starts in the previous block); blk receives the block index. */
+typedef struct {
+    uint32_t dm;
+    int32_t t;
+    uint32_t width;
+    float snr;
+    uint32_t blk;
+} orc_scand;
+int orc_boxcar_stream(const float *x, int ndms, int n, int nblk, int64_t stride,
+                      const int *widths, int nwidths, double thr, orc_scand *cands, int cap);
 
 /* --- reference in-path detector (median of five) --------------------- */
 float orc_median_of_five(float a, float b, float c, float d, float e);

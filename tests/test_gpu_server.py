@@ -62,32 +62,54 @@ def _run(srv, beams_and_streams, N, nblocks, interleave=True, hold_first=False):
     return res
 
 
-def _oracle_beam(x, beam, N, nblocks, ranges, C):
-    """Oracle pipeline over the stream: clip each block's new samples, then
-    dedisperse the zero-history + stream series and detect per block."""
+def _oracle_beam(x, beam, N, nblocks, ranges, C, clip=5.0):
+    """Oracle pipeline over the stream: clip each block's new samples,
+    dedisperse the zero-history + stream series block by block, then search
+    each range's series -- the first N/k samples of every block's rows,
+    concatenated -- with the streaming detector (every window once, per-block
+    statistics).  Returns per block (packed output, candidates)."""
     M = beam.overlap
     clipped = np.empty_like(x)
     for k in range(nblocks):
-        out, *_ = oracle.clip_u8(x[:, k * N:(k + 1) * N], 0, N, 5.0)
+        if clip > 0:
+            out, *_ = oracle.clip_u8(x[:, k * N:(k + 1) * N], 0, N, clip)
+        else:
+            out = x[:, k * N:(k + 1) * N]
         clipped[:, k * N:(k + 1) * N] = out
     full = np.concatenate([np.zeros((C, M), np.uint8), clipped], axis=1)
-    per_block = []
+    outs = [[] for _ in range(nblocks)]
+    series = [[] for _ in ranges]
     for k in range(nblocks):
         seg = np.ascontiguousarray(full[:, k * N:k * N + M + N])
-        outs, cands = [], []
-        first = 0
-        for (lo, step, nd, kk) in ranges:
+        for r, (lo, step, nd, kk) in enumerate(ranges):
             ts = np.float32(64e-6) * np.float32(kk)
             sh = oracle.shift_table_ref(1500.0, -300.0 / C, C, lo, step, nd, ts)
             nout = (M + N) // kk - int(sh.max())
             o = oracle.range_f32(seg, kk, sh, nout)
-            outs.append(o.reshape(-1))
-            c, _, _ = oracle.boxcar_detect(np.ascontiguousarray(o[:, :N // kk]), list(WIDTHS), 6.0)
-            c["dm"] += first
-            cands.append(c)
-            first += nd
-        per_block.append((np.concatenate(outs), np.concatenate(cands)))
-    return per_block
+            outs[k].append(o.reshape(-1))
+            series[r].append(o[:, :N // kk])
+    cands = [[] for _ in range(nblocks)]
+    first = 0
+    for r, (lo, step, nd, kk) in enumerate(ranges):
+        c = oracle.boxcar_stream(np.concatenate(series[r], axis=1), N // kk, list(WIDTHS), 6.0)
+        c["dm"] += first
+        for k in range(nblocks):
+            sel = c[c["blk"] == k]
+            cands[k].append(np.array(sel[["dm", "t", "width", "snr"]].tolist(),
+                                     dtype=[("dm", np.uint32), ("t", np.int32),
+                                            ("width", np.uint32), ("snr", np.float32)]))
+        first += nd
+    return [(np.concatenate(outs[k]), np.concatenate(cands[k])) for k in range(nblocks)]
+
+
+def _overlap(C, N, ranges):
+    """The overlap M a beam of this shape keeps (mdsm_beam_info)."""
+    from mdsm_amd.server import Server
+    srv = Server()
+    b = srv.open_beam(**_beam_kwargs(C, N, ranges))
+    M = b.overlap
+    srv.close()
+    return M
 
 
 def _key(c):
@@ -100,7 +122,13 @@ def test_alone_and_in_mix_identical_and_match_oracle(gpu):
     C, N, nblocks = 64, 4096, 3
     ranges = ((0.0, 2.0, 40, 1), (80.0, 4.0, 20, 2))
     print("case server alone/mix", flush=True)
-    streams = [_stream(M, C, N, nblocks, 100 + i, [(30.0 + 20 * i, 2000 + 3000 * i, 4, 3.0)])
+    Mo = _overlap(C, N, ranges)
+    # pulses inside blocks and across block edges: output sample t of block k
+    # is stream sample k*N - M + t, so a pulse arriving (top of the band) at
+    # k*N - M - 6 lies across the edge of blocks k-1 and k in the series
+    streams = [_stream(M, C, N, nblocks, 100 + i, [(30.0 + 20 * i, 2000 + 3000 * i, 4, 3.0),
+                                                   (25.0 + 10 * i, N - Mo - 6 + i, 12, 2.0),
+                                                   (60.0, 2 * N - Mo - 3, 32, 1.5)])
                for i in range(4)]
     for x in streams:          # impulsive RFI for the clipper
         x[5, 100:110] = 255
@@ -128,6 +156,8 @@ def test_alone_and_in_mix_identical_and_match_oracle(gpu):
             assert np.array_equal(out_m, ref[k][0]), f"beam {i} block {k}: output != oracle"
             assert [t[:3] for t in _key(c_m)] == [t[:3] for t in _key(ref[k][1])], \
                 f"beam {i} block {k}: candidates != oracle"
+    # windows that start in the previous block were searched (t < 0)
+    assert any(int(t) < 0 for v in mix.values() for t in v[1]["t"])
     st = srv.stats()
     assert st["completed"] == len(streams) * nblocks and st["launches"] < st["completed"]
     srv.close()
