@@ -183,6 +183,17 @@ typedef struct {
                              (0 or 1: every block of a tile, then the next tile) */
 } mdsm_tuning;
 
+/* Occupancy of the dedispersion kernel stage `stage` of the plan launches
+ * with its current tuning (theoretical, from the kernel's register and LDS
+ * use): resident workgroups per compute unit, waves per SIMD (4 SIMDs per
+ * CU), VGPRs per lane, static and dynamic LDS bytes per workgroup, scratch
+ * bytes per lane.  No counterpart in the reference. */
+typedef struct {
+    int32_t workgroups_per_cu, waves_per_simd, vgprs, static_lds, dynamic_lds, scratch_bytes;
+} mdsm_occupancy;
+
+int mdsm_plan_occupancy(mdsm_plan *plan, uint32_t stage, mdsm_occupancy *out);
+
 int mdsm_plan_set_tuning(mdsm_plan *plan, const mdsm_tuning *t);
 int mdsm_plan_get_tuning(const mdsm_plan *plan, mdsm_tuning *t);
 

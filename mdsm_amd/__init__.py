@@ -112,6 +112,12 @@ CAND_DTYPE = np.dtype([("dm", np.uint32), ("t", np.int32), ("width", np.uint32),
                        ("snr", np.float32)])
 
 
+class Occupancy(C.Structure):
+    _fields_ = [("workgroups_per_cu", C.c_int32), ("waves_per_simd", C.c_int32),
+                ("vgprs", C.c_int32), ("static_lds", C.c_int32), ("dynamic_lds", C.c_int32),
+                ("scratch_bytes", C.c_int32)]
+
+
 class Pulse(C.Structure):
     _fields_ = [("dm", C.c_double), ("t0", C.c_uint32), ("width", C.c_uint32),
                 ("amp", C.c_float), ("pad", C.c_uint32)]
@@ -186,6 +192,7 @@ def lib():
         "mdsm_batch_autotune": [vp, C.POINTER(vp), C.POINTER(i64), C.POINTER(vp), C.POINTER(vp),
                                 vp, u32, C.POINTER(Tuning)],
         "mdsm_plan_set_tuning": [vp, C.POINTER(Tuning)],
+        "mdsm_plan_occupancy": [vp, u32, C.POINTER(Occupancy)],
         "mdsm_plan_get_tuning": [vp, C.POINTER(Tuning)],
         "mdsm_plan_autotune": [vp, vp, i64, vp, vp, vp, u32, C.POINTER(Tuning)],
         "mdsm_device_count": [C.POINTER(C.c_int)],
@@ -459,6 +466,12 @@ class Plan:
         check(lib().mdsm_plan_autotune(self._h, pin, int(in_stride), pout, pw, _sh(stream),
                                        int(reps), C.byref(t)))
         return t.as_dict()
+
+    def occupancy(self, stage: int = 0) -> dict:
+        """Theoretical occupancy of the dedispersion launch of `stage`."""
+        o = Occupancy()
+        check(lib().mdsm_plan_occupancy(self._h, int(stage), C.byref(o)))
+        return {f: getattr(o, f) for f, _ in Occupancy._fields_}
 
     def split_output(self, flat: np.ndarray) -> list:
         return [flat[R.out_offset:R.out_offset + R.ndms * R.nout].reshape(R.ndms, R.nout)

@@ -891,6 +891,29 @@ hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s)
     return hipGetLastError();
 }
 
+hipError_t dedisperse_occupancy(int32_t lds_bytes, int32_t dtypes, DedispOccupancy *o)
+{
+    const bool ints = (dtypes & ((1 << DT_U8) | (1 << DT_U16))) != 0;
+    const bool floats = (dtypes & ((1 << DT_F16) | (1 << DT_F32))) != 0;
+    const void *kern = (floats && !ints) ? (const void *)dedisperse_kernel<kClsFloat>
+                                         : (const void *)dedisperse_kernel<kClsAll>;
+    hipFuncAttributes at{};
+    hipError_t e = hipFuncGetAttributes(&at, kern);
+    if (e != hipSuccess) return e;
+    e = hipFuncSetAttribute(kern, hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax);
+    if (e != hipSuccess) return e;
+    int nb = 0;
+    e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, kern, kThreads, (size_t)lds_bytes);
+    if (e != hipSuccess) return e;
+    o->workgroups_per_cu = nb;
+    o->waves_per_simd = nb * kWaves / 4;
+    o->vgprs = at.numRegs;
+    o->static_lds = (int32_t)at.sharedSizeBytes;
+    o->dynamic_lds = lds_bytes;
+    o->scratch_bytes = (int32_t)at.localSizeBytes;
+    return hipSuccess;
+}
+
 hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in,
                              int64_t in_stride, float *out, void *work, hipStream_t s)
 {

@@ -238,6 +238,12 @@ hipError_t launch_plain(const float *in, int64_t in_stride, uint32_t nchans,
 hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in,
                              int64_t in_stride, float *out, void *work, hipStream_t s);
 hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s);
+// Resident workgroups per CU (and waves per SIMD) of the dedispersion kernel
+// a launch with this dynamic LDS and these dtypes takes, with its registers.
+struct DedispOccupancy {
+    int32_t workgroups_per_cu, waves_per_simd, vgprs, static_lds, dynamic_lds, scratch_bytes;
+};
+hipError_t dedisperse_occupancy(int32_t lds_bytes, int32_t dtypes, DedispOccupancy *o);
 hipError_t launch_decimate(const void *in, int dtype_in, uint32_t nchans, int64_t in_stride,
                            uint32_t nsamp, uint32_t k, void *out, int dtype_out,
                            int64_t out_stride, hipStream_t s);

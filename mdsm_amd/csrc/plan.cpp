@@ -819,6 +819,23 @@ extern "C" int mdsm_dedisperse(mdsm_plan *p, const void *d_in, int64_t in_stride
     return MDSM_OK;
 }
 
+extern "C" int mdsm_plan_occupancy(mdsm_plan *p, uint32_t stage, mdsm_occupancy *out)
+{
+    if (!p || !out) return fail(MDSM_ERR_INVALID, "NULL argument");
+    if (stage >= p->stages.size()) return fail(MDSM_ERR_INVALID, "no such stage");
+    const Stage &st = p->stages[stage];
+    DedispOccupancy o{};
+    hipError_t e = dedisperse_occupancy(st.lds_bytes, st.dtypes, &o);
+    if (e != hipSuccess) return hip_fail(e, "occupancy query");
+    out->workgroups_per_cu = o.workgroups_per_cu;
+    out->waves_per_simd = o.waves_per_simd;
+    out->vgprs = o.vgprs;
+    out->static_lds = o.static_lds;
+    out->dynamic_lds = o.dynamic_lds;
+    out->scratch_bytes = o.scratch_bytes;
+    return MDSM_OK;
+}
+
 extern "C" int mdsm_plan_set_tuning(mdsm_plan *p, const mdsm_tuning *t)
 {
     if (!p || !t) return fail(MDSM_ERR_INVALID, "NULL argument");

@@ -209,6 +209,12 @@ struct mdsm_server {
     bool held = false;  // admissions paused (mdsm_server_hold)
     hipStream_t streams[kStreams] = {nullptr, nullptr};
     uint32_t *counters = nullptr;  // persistent-mode work counters, one per stream
+    // end of the latest dedispersion launch: the next one (on the other
+    // stream) waits for it, so dedispersion launches never share the GPU and
+    // each launch's duration is its own, while the small kernels and copies
+    // of one stream still overlap the other's dedispersion
+    hipEvent_t ded_done = nullptr;
+    bool ded_recorded = false;
     int next_stream = 0;
     mdsm_server_stats st{};
     std::string sched_error;
@@ -400,6 +406,10 @@ int launch_batch(mdsm_server *s, Batch *bt)
     hipError_t e = hipMemcpyAsync(bt->segs_dev, bt->segs_host.data(),
                                   bt->segs_host.size() * sizeof(SegDesc), hipMemcpyHostToDevice, st);
     if (e != hipSuccess) return hip_fail(e, "server segment upload");
+    if (s->ded_recorded) {
+        e = hipStreamWaitEvent(st, s->ded_done, 0);
+        if (e != hipSuccess) return hip_fail(e, "server launch order");
+    }
     for (LaunchSpec &L : specs) {
         L.segs = (const SegDesc *)bt->segs_dev;
         TimedLaunch tl("dedisperse", st);
@@ -407,6 +417,9 @@ int launch_batch(mdsm_server *s, Batch *bt)
         if (e != hipSuccess) return hip_fail(e, "server dedisperse");
         s->st.launches++;
     }
+    e = hipEventRecord(s->ded_done, st);
+    if (e != hipSuccess) return hip_fail(e, "server launch order");
+    s->ded_recorded = true;
     // 3. overlap for the next block, detection, results back
     for (Job *j : bt->jobs) {
         Beam *b = j->beam;
@@ -731,6 +744,11 @@ extern "C" int mdsm_server_create(const mdsm_server_config *cfg, mdsm_server **o
         mdsm_server_destroy(s);
         return hip_fail(e, "server counters");
     }
+    e = hipEventCreateWithFlags(&s->ded_done, hipEventDisableTiming);
+    if (e != hipSuccess) {
+        mdsm_server_destroy(s);
+        return hip_fail(e, "server event");
+    }
     s->sched = std::thread(scheduler, s);
     *out = s;
     return MDSM_OK;
@@ -759,6 +777,7 @@ extern "C" void mdsm_server_destroy(mdsm_server *s)
     for (auto &kv : s->beams) free_beam(s, kv.second);
     s->pool.clear();
     if (s->counters) (void)hipFree(s->counters);
+    if (s->ded_done) (void)hipEventDestroy(s->ded_done);
     for (auto &st : s->streams)
         if (st) (void)hipStreamDestroy(st);
     delete s;

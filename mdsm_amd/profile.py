@@ -177,7 +177,7 @@ def read_traffic(path: str) -> dict | None:
         with open(path) as f:
             out = {}
             for line in f:
-                if "_bytes_per_launch = " in line:
+                if "_bytes_per_launch = " in line or "_bytes_per_block = " in line:
                     k, v = line.split(" = ")
                     out[k.strip()] = float(v)
             return out or None
