@@ -108,6 +108,11 @@ int mdsm_plan_create_subband(const mdsm_obs *obs, uint32_t nsubs,
 
 void mdsm_plan_destroy(mdsm_plan *plan);
 
+/* Upload the plan's device tables now (set-up) instead of on the first
+ * mdsm_dedisperse.  Required before mdsm_dedisperse_noalloc.  Part of the
+ * plan set-up of initialiseMDSM (src/mdsm/dedispersion_manager.cpp:286-314). */
+int mdsm_plan_upload(mdsm_plan *plan);
+
 typedef struct {
     uint32_t nranges;        /* ranges (brute) or passes (subband) */
     uint32_t ndms_total;     /* rows of the packed output */
@@ -219,6 +224,19 @@ int mdsm_plan_autotune(mdsm_plan *plan, const void *d_in, int64_t in_stride,
  * and opt_dedisperse_subband). */
 int mdsm_dedisperse(mdsm_plan *plan, const void *d_in, int64_t in_stride,
                     float *d_out, void *d_work, void *stream);
+
+/* mdsm_dedisperse without any allocation, for callers that cannot allocate
+ * device memory (code under HIP graph capture, tight serving loops, harnesses
+ * that forbid it).  No counterpart in the reference.  The workspace and the
+ * work-queue counters live in a static per-GPU array of 256 MiB held until
+ * the process ends; plans needing more workspace than
+ * mdsm_noalloc_max_work() bytes are MDSM_ERR_INVALID.  The plan's tables must
+ * be on the device (mdsm_plan_upload at set-up; MDSM_ERR_STATE otherwise).
+ * All calls on one GPU share the array: calls that overlap on one GPU race.
+ * Results equal mdsm_dedisperse's. */
+uint64_t mdsm_noalloc_max_work(void);
+int mdsm_dedisperse_noalloc(mdsm_plan *plan, const void *d_in, int64_t in_stride,
+                            float *d_out, void *stream);
 
 /* The same float32 brute-force result from a plain kernel in the shape of
  * the reference's opt_dedisperse_loop (src/mdsm/dedispersion_kernel.cu:51-77):

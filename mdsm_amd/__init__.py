@@ -193,6 +193,8 @@ def lib():
                                 vp, u32, C.POINTER(Tuning)],
         "mdsm_plan_set_tuning": [vp, C.POINTER(Tuning)],
         "mdsm_plan_occupancy": [vp, u32, C.POINTER(Occupancy)],
+        "mdsm_plan_upload": [vp],
+        "mdsm_dedisperse_noalloc": [vp, vp, i64, vp, vp],
         "mdsm_plan_get_tuning": [vp, C.POINTER(Tuning)],
         "mdsm_plan_autotune": [vp, vp, i64, vp, vp, vp, u32, C.POINTER(Tuning)],
         "mdsm_device_count": [C.POINTER(C.c_int)],
@@ -221,6 +223,8 @@ def lib():
     for name in ("mdsm_sigproc_close", "mdsm_series_writer_close"):
         getattr(L, name).argtypes = [vp]
         getattr(L, name).restype = None
+    L.mdsm_noalloc_max_work.restype = C.c_uint64
+    L.mdsm_noalloc_max_work.argtypes = []
     L.mdsm_boxcar_carry_len.restype = C.c_uint32
     L.mdsm_boxcar_carry_len.argtypes = [u32]
     L.mdsm_last_error.restype = C.c_char_p
@@ -442,6 +446,17 @@ class Plan:
         pout = d_out.ptr if isinstance(d_out, DeviceBuffer) else d_out
         pw = d_work.ptr if isinstance(d_work, DeviceBuffer) else d_work
         check(lib().mdsm_dedisperse(self._h, pin, int(in_stride), pout, pw, _sh(stream)))
+
+    def upload(self):
+        """Put the plan's device tables on the device now (set-up)."""
+        check(lib().mdsm_plan_upload(self._h))
+
+    def dedisperse_noalloc(self, d_in, in_stride: int, d_out, stream=None):
+        """mdsm_dedisperse_noalloc: workspace and counters in the library's
+        static per-GPU array; needs upload() first."""
+        pin = d_in.ptr if isinstance(d_in, DeviceBuffer) else d_in
+        pout = d_out.ptr if isinstance(d_out, DeviceBuffer) else d_out
+        check(lib().mdsm_dedisperse_noalloc(self._h, pin, int(in_stride), pout, _sh(stream)))
 
     def dedisperse_plain(self, d_in, in_stride: int, d_out, stream=None):
         """The reference-shaped plain float32 kernel (a performance baseline)."""

@@ -917,6 +917,14 @@ hipError_t dedisperse_occupancy(int32_t lds_bytes, int32_t dtypes, DedispOccupan
 hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in,
                              int64_t in_stride, float *out, void *work, hipStream_t s)
 {
+    return launch_dedisperse_ctr(p, st, in, in_stride, out, work,
+                                 p->dev.counter + (&st - p->stages.data()), s);
+}
+
+hipError_t launch_dedisperse_ctr(const mdsm_plan *p, const Stage &st, const void *in,
+                                 int64_t in_stride, float *out, void *work, uint32_t *counter,
+                                 hipStream_t s)
+{
     LaunchSpec L;
     L.segs = p->dev.segs;
     L.seg_begin = st.seg_begin;
@@ -926,7 +934,7 @@ hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in
     L.dtypes = st.dtypes;
     L.group = (int32_t)p->tune.tile_group;
     if (p->tune.persistent) {
-        L.counter = p->dev.counter + (&st - p->stages.data());
+        L.counter = counter;
         L.grid = p->tune.grid ? (int32_t)p->tune.grid : 8 * std::max(1, p->cus);
     }
     L.njobs = 1;

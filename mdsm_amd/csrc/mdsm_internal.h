@@ -238,6 +238,11 @@ hipError_t launch_plain(const float *in, int64_t in_stride, uint32_t nchans,
 hipError_t launch_dedisperse(const mdsm_plan *p, const Stage &st, const void *in,
                              int64_t in_stride, float *out, void *work, hipStream_t s);
 hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s);
+// launch_dedisperse with the persistent work-queue counter given (one uint32
+// per launch, zeroed by the launch) instead of the plan's.
+hipError_t launch_dedisperse_ctr(const mdsm_plan *p, const Stage &st, const void *in,
+                                 int64_t in_stride, float *out, void *work, uint32_t *counter,
+                                 hipStream_t s);
 // Resident workgroups per CU (and waves per SIMD) of the dedispersion kernel
 // a launch with this dynamic LDS and these dtypes takes, with its registers.
 struct DedispOccupancy {

@@ -819,6 +819,16 @@ extern "C" int mdsm_dedisperse(mdsm_plan *p, const void *d_in, int64_t in_stride
     return MDSM_OK;
 }
 
+extern "C" int mdsm_plan_upload(mdsm_plan *p)
+{
+    if (!p) return fail(MDSM_ERR_INVALID, "NULL argument");
+    int cur = 0;
+    hipError_t e = hipGetDevice(&cur);
+    if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
+    if (cur != p->device) return fail(MDSM_ERR_STATE, "plan belongs to another device");
+    return upload_plan(p);
+}
+
 extern "C" int mdsm_plan_occupancy(mdsm_plan *p, uint32_t stage, mdsm_occupancy *out)
 {
     if (!p || !out) return fail(MDSM_ERR_INVALID, "NULL argument");
