@@ -32,10 +32,12 @@ import numpy as np
 
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 VALU_ADD_PEAK_T = 39.3         # unpacked fp32 adds/s (256 CU x 2.4 GHz x 64), T/s
-# LDS read peak: 128 B/clk/CU x 256 CUs x 2.4 GHz.  A ds_read_b64 probe on
-# MI355X reaches 75.4 TB/s (122.8 B/clk/CU) and ds_read_b128 121.5 TB/s.
-LDS_PEAK_TBS = 78.6
-LDS_PEAK_SOURCE = "128 B/clk/CU x 256 CUs x 2.4 GHz"
+# LDS read roof, measured on MI355X (profiles/lds_probe.md, mdsm_probe_lds_read):
+# ds_read_b128 at 8 waves per SIMD 137.6 TB/s; ds_read_b64, the integer
+# kernel's read, 98.5 TB/s at 4 waves per SIMD and 109 TB/s at 8.
+LDS_PEAK_TBS = 137.6
+LDS_PEAK_SOURCE = "profiles/lds_probe.md: ds_read_b128, 8 waves per SIMD"
+LDS_B64_AT_4_WAVES_TBS = 98.5
 # BASELINE's headline metric: throughput, its fraction of the HBM roofline,
 # real-time factor and p99 block latency, against the plain float32 kernel
 BASELINE_METRIC = ("DM\u00b7chan\u00b7samples/s, %HBM roofline, RT factor, p99 block latency "
@@ -168,6 +170,9 @@ def serve(M, w, args):
     st_thr = srv.stats()
     nblk = args.steps * B
     assert len(results) == nblk
+    if getattr(args, "throughput_only", False):   # the profiler's program: timed phase only
+        srv.close()
+        return {"blocks": nblk + args.warmup * B, "seconds": wall}
     kern = {k: M.timing_get(k) for k in ("dedisperse", "clip", "detect", "decimate")}
     value = nblk * units_blk / wall
     rt_factor = nblk * block_s / wall
@@ -179,7 +184,7 @@ def serve(M, w, args):
     ded_s_blk = ded_ms / 1e3 / nblk
     achieved = alg_blk / ded_s_blk / 1e9
     from mdsm_amd.profile import read_traffic
-    prof = os.path.join(PROFILE_DIR, "dedisperse_kernel_cfg5_served.md")
+    prof = os.path.join(PROFILE_DIR, "served_pipeline_cfg5.md")
     tr = read_traffic(prof)
     lds_bytes_blk = units_blk * 2.0
     occ = {"occupancy_theoretical": tp.occupancy(0)}
@@ -285,7 +290,8 @@ def serve(M, w, args):
         "lds": {"bytes_per_block": lds_bytes_blk,
                 "achieved_tbs": lds_bytes_blk / ded_s_blk / 1e12, "peak_tbs": LDS_PEAK_TBS,
                 "frac": lds_bytes_blk / ded_s_blk / 1e12 / LDS_PEAK_TBS,
-                "peak_source": LDS_PEAK_SOURCE, **occ},
+                "peak_source": LDS_PEAK_SOURCE, "b64_at_4_waves_tbs": LDS_B64_AT_4_WAVES_TBS,
+                **occ},
         "kernels_ms_per_block": {k: v[0] / nblk for k, v in kern.items()},
         "mean_batch": mean_batch_thr, "pulse_blocks": pulses_thr, "tuning": tuning,
         "server_stats": {k: int(v) for k, v in st.items()},

@@ -712,6 +712,14 @@ int mdsm_timing_enable(int on);
 int mdsm_timing_reset(void);
 int mdsm_timing_get(const char *kernel, double *total_ms, uint64_t *launches);
 
+/* LDS read bandwidth of this GPU (TB/s) for reads of bytes_per_lane = 4, 8
+ * or 16 bytes per lane (ds_read_b32/_b64/_b128, 64 contiguous lanes, a
+ * wave-uniform offset moving on every read: the trial reads of the
+ * dedispersion kernel) at waves_per_simd (1..8) resident waves per SIMD.
+ * The roof the dedispersion kernel's LDS rate is measured against.  No
+ * counterpart in the reference. */
+int mdsm_probe_lds_read(uint32_t bytes_per_lane, uint32_t waves_per_simd, double *tbs);
+
 const char *mdsm_last_error(void);
 const char *mdsm_version(void);
 

@@ -194,6 +194,7 @@ def lib():
         "mdsm_plan_set_tuning": [vp, C.POINTER(Tuning)],
         "mdsm_plan_occupancy": [vp, u32, C.POINTER(Occupancy)],
         "mdsm_plan_upload": [vp],
+        "mdsm_probe_lds_read": [u32, u32, C.POINTER(f64)],
         "mdsm_dedisperse_noalloc": [vp, vp, i64, vp, vp],
         "mdsm_plan_get_tuning": [vp, C.POINTER(Tuning)],
         "mdsm_plan_autotune": [vp, vp, i64, vp, vp, vp, u32, C.POINTER(Tuning)],
@@ -682,6 +683,13 @@ def detect_boxcar(d_x, ndms, n, stride, widths, thr, d_cands, cap, d_count, d_me
     w = (C.c_uint32 * len(widths))(*widths)
     check(lib().mdsm_detect_boxcar(g(d_x), ndms, n, stride, w, len(widths), thr, g(d_cands), cap,
                                    g(d_count), g(d_mean), g(d_sigma), _sh(stream)))
+
+
+def probe_lds_read(bytes_per_lane: int, waves_per_simd: int) -> float:
+    """Measured LDS read bandwidth (TB/s) of the current GPU."""
+    v = C.c_double()
+    check(lib().mdsm_probe_lds_read(int(bytes_per_lane), int(waves_per_simd), C.byref(v)))
+    return v.value
 
 
 def boxcar_carry_len(widths) -> int:

@@ -1,0 +1,98 @@
+// LDS read-bandwidth probe: the roof the dedispersion kernel is measured
+// against.  Each wave reads 64 lanes x `W` contiguous bytes per instruction
+// (ds_read_b32 / _b64 / _b128) at a wave-uniform offset that moves on every
+// read, the access shape of the dedispersion kernel's trial reads; the offset
+// is masked with a run-time mask so that the compiler cannot merge reads.
+// Every address is a multiple of 16 bytes (of W for the lane part), so each
+// read is naturally aligned and ends inside the masked region.
+// The number of resident waves per SIMD is set through the workgroup's LDS
+// allocation (one wave per SIMD per workgroup of 256 threads).
+#include <hip/hip_runtime.h>
+
+#include "mdsm_internal.h"
+
+using namespace mdsm;
+
+namespace {
+
+constexpr int kProbeUnroll = 16;
+
+template <int W>
+__global__ void __launch_bounds__(256) lds_probe_kernel(int iters, uint32_t mask, uint32_t *out)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    for (uint32_t i = threadIdx.x; i < (mask + 1) / 4; i += 256)
+        ((uint32_t *)lds)[i] = i * 2654435761u;
+    __syncthreads();
+    uint32_t acc = 0;
+    uint32_t off = wave * 1024u;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int u = 0; u < kProbeUnroll; ++u) {
+            const uint32_t a = (off + (uint32_t)u * 528u + lane * W) & mask;
+            if constexpr (W == 4) {
+                acc += *(const uint32_t *)(lds + a);
+            } else if constexpr (W == 8) {
+                const uint2 v = *(const uint2 *)(lds + a);
+                acc += v.x ^ v.y;
+            } else {
+                const uint4 v = *(const uint4 *)(lds + a);
+                acc += (v.x ^ v.y) + (v.z ^ v.w);
+            }
+        }
+        off += kProbeUnroll * 528u + 16u;
+    }
+    out[blockIdx.x * 256 + threadIdx.x] = acc;
+}
+
+}  // namespace
+
+extern "C" int mdsm_probe_lds_read(uint32_t bytes_per_lane, uint32_t waves_per_simd, double *tbs)
+{
+    if (!tbs) return fail(MDSM_ERR_INVALID, "NULL argument");
+    if (bytes_per_lane != 4 && bytes_per_lane != 8 && bytes_per_lane != 16)
+        return fail(MDSM_ERR_INVALID, "bytes_per_lane must be 4, 8 or 16");
+    if (waves_per_simd < 1 || waves_per_simd > 8)
+        return fail(MDSM_ERR_INVALID, "waves_per_simd must be 1..8");
+    int dev = 0, cus = 256;
+    hipError_t e = hipGetDevice(&dev);
+    if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (e != hipSuccess) return hip_fail(e, "device query");
+    // LDS per workgroup: the CU's 160 KiB shared by waves_per_simd workgroups,
+    // rounded down to a power of two for the address mask (>= 16 KiB)
+    uint32_t lds = 16384;
+    while (lds * 2 * waves_per_simd <= 163840u - 4096u && lds < 65536) lds *= 2;
+    const uint32_t need = 163840u / waves_per_simd - 1024u;  // occupancy set by allocation
+    const uint32_t alloc = need > lds ? need : lds;
+    const uint32_t mask = lds - 1;
+    const int grid = cus * (int)waves_per_simd * 8;
+    const int iters = 512;
+    uint32_t *out = nullptr;
+    e = hipMalloc(&out, (size_t)grid * 256 * 4);
+    if (e != hipSuccess) return hip_fail(e, "probe buffer");
+    void (*k)(int, uint32_t, uint32_t *) =
+        bytes_per_lane == 4 ? lds_probe_kernel<4>
+                            : (bytes_per_lane == 8 ? lds_probe_kernel<8> : lds_probe_kernel<16>);
+    e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    if (e == hipSuccess) e = hipEventCreate(&e0);
+    if (e == hipSuccess) e = hipEventCreate(&e1);
+    float ms = 0.0f;
+    if (e == hipSuccess) {
+        hipLaunchKernelGGL(k, dim3(grid), dim3(256), alloc, 0, iters, mask, out);  // warm-up
+        (void)hipEventRecord(e0, 0);
+        hipLaunchKernelGGL(k, dim3(grid), dim3(256), alloc, 0, iters, mask, out);
+        (void)hipEventRecord(e1, 0);
+        e = hipGetLastError();
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    }
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipFree(out);
+    if (e != hipSuccess) return hip_fail(e, "LDS probe");
+    const double bytes = (double)grid * 256.0 * iters * kProbeUnroll * bytes_per_lane;
+    *tbs = bytes / (ms * 1e-3) / 1e12;
+    return MDSM_OK;
+}

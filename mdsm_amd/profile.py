@@ -51,6 +51,28 @@ def _steps(workload: str, steps: int, tile_group=None) -> None:
         print(f"step {i + 1}/{steps} done", flush=True)
 
 
+def _serve(steps: int, warmup: int) -> None:
+    """The timed phase of the served config-5 bench (32 beams, HBM-resident
+    blocks pushed back to back): warmup + steps blocks per beam."""
+    sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+    import mdsm_amd as M
+    import bench
+    ns = argparse.Namespace(steps=steps, warmup=warmup, throughput_only=True)
+    r = bench.serve(M, bench.WORKLOADS["cfg5"], ns)
+    print(f"served blocks {r['blocks']} in the profiled run", flush=True)
+
+
+def _lds_probe() -> None:
+    """LDS read bandwidth of ds_read_b32/_b64/_b128 at 1, 2, 4 and 8 waves per SIMD."""
+    import mdsm_amd as M
+    print("| bytes per lane | waves per SIMD | TB/s |")
+    print("|---|---|---|")
+    for w in (4, 8, 16):
+        for occ in (1, 2, 4, 8):
+            v = max(M.probe_lds_read(w, occ) for _ in range(3))
+            print(f"| {w} | {occ} | {v:.1f} |", flush=True)
+
+
 def _copy() -> None:
     import mdsm_amd as M
     a = M.DeviceBuffer(COPY_C * COPY_S * 4)
@@ -110,14 +132,19 @@ def summarize(args) -> None:
     print(f"# rocprofv3 summary: {args.kernel}, workload {args.workload}\n")
     print("Commands (repository root, one GPU):\n")
     print("```")
-    prog = f"python3 -m mdsm_amd.profile steps --workload {args.workload}"
-    print(f"rocprofv3 --kernel-trace --stats -d {args.dir}/kt -o run -- {prog} --steps 3")
-    print(f"rocprofv3 --pmc FETCH_SIZE -d {args.dir}/fetch -o run -- {prog} --steps 1")
-    print(f"rocprofv3 --pmc WRITE_SIZE -d {args.dir}/write -o run -- {prog} --steps 1")
+    if args.workload == "cfg5":
+        prog = "python3 -m mdsm_amd.profile serve"
+        kt_args, pmc_args = "--steps 4 --warmup 1", "--steps 1 --warmup 1"
+    else:
+        prog = f"python3 -m mdsm_amd.profile steps --workload {args.workload}"
+        kt_args, pmc_args = "--steps 3", "--steps 1"
+    print(f"rocprofv3 --kernel-trace --stats -d {args.dir}/kt -o run -- {prog} {kt_args}")
+    print(f"rocprofv3 --pmc FETCH_SIZE -d {args.dir}/fetch -o run -- {prog} {pmc_args}")
+    print(f"rocprofv3 --pmc WRITE_SIZE -d {args.dir}/write -o run -- {prog} {pmc_args}")
     print(f"rocprofv3 --pmc FETCH_SIZE -d {args.dir}/cfetch -o run -- python3 -m mdsm_amd.profile copy")
     print(f"rocprofv3 --pmc WRITE_SIZE -d {args.dir}/cwrite -o run -- python3 -m mdsm_amd.profile copy")
     print(f"python3 -m mdsm_amd.profile summarize --dir {args.dir} --workload {args.workload} "
-          f"--kernel {args.kernel}")
+          f"--kernel {args.kernel}" + (f" --blocks {args.blocks}" if args.blocks else ""))
     print("```\n")
     print("## Kernel statistics (--kernel-trace --stats)\n")
     print("| kernel | calls | total ms | average ms | % |")
@@ -147,15 +174,23 @@ def summarize(args) -> None:
             if v:
                 vals[n] = sum(v) / len(v)
         print("\n## LDS and occupancy counters per dispatch (--pmc, one run)\n")
-        print(f"`rocprofv3 --pmc {' '.join(names)} -d {args.dir}/lds -o run -- {prog} --steps 1`\n")
+        print(f"`rocprofv3 --pmc {' '.join(names)} -d {args.dir}/lds -o run -- {prog} {pmc_args}`\n")
         for n in names:
             if n in vals:
                 print(f"* {n} = {vals[n]:.4e}")
         if "SQ_INSTS_LDS_LOAD_BANDWIDTH" in vals and d:
             lb = vals["SQ_INSTS_LDS_LOAD_BANDWIDTH"] * 64.0
             avg_s = sum(r[0] for r in d) / len(d) / 1e9
-            print(f"* LDS load bytes (64-byte units x 64) = {lb:.4e}; at the traced average "
-                  f"duration that is {lb / avg_s / 1e12:.1f} TB/s")
+            if args.blocks:   # dispatches of the served run carry different numbers of blocks
+                ltot = sum(counter_per_dispatch(ldb, args.kernel, "SQ_INSTS_LDS_LOAD_BANDWIDTH"))
+                kt_tot = sum(r[0] for r in d) / 1e9
+                kt_blocks = args.kt_blocks or args.blocks
+                print(f"* LDS load bytes per block = {ltot * 64.0 / args.blocks:.4e}; at the "
+                      f"traced kernel time per block ({kt_tot / kt_blocks * 1e3:.3f} ms) that is "
+                      f"{ltot * 64.0 / args.blocks / (kt_tot / kt_blocks) / 1e12:.1f} TB/s")
+            else:
+                print(f"* LDS load bytes (64-byte units x 64) = {lb:.4e}; at the traced average "
+                      f"duration that is {lb / avg_s / 1e12:.1f} TB/s")
         if "SQ_LDS_BANK_CONFLICT" in vals and vals.get("SQ_LDS_IDX_ACTIVE"):
             print(f"* bank-conflict cycles / LDS-active cycles = "
                   f"{vals['SQ_LDS_BANK_CONFLICT'] / vals['SQ_LDS_IDX_ACTIVE']:.4f}")
@@ -166,6 +201,20 @@ def summarize(args) -> None:
             w = vals["SQ_WAVE_CYCLES"] * 4.0 / cyc / 256.0
             print(f"* mean resident waves per CU = SQ_WAVE_CYCLES x 4 / (GRBM_GUI_ACTIVE / 8) / 256 "
                   f"= {w:.1f} ({w / 4:.2f} per SIMD)")
+    if args.blocks:
+        # the served run's dispatches carry different numbers of blocks: per block
+        fb = sum(fk) * 1024.0 * f_scale / args.blocks
+        wb = sum(wk) * 1024.0 * w_scale / args.blocks
+        print(f"* over the {args.blocks} blocks of the counter runs: {fb / 1e9:.4f} GB read and "
+              f"{wb / 1e9:.4f} GB written per block")
+        if d:
+            kt_blocks = args.kt_blocks or args.blocks
+            print(f"* traced `{args.kernel}` time per block: "
+                  f"{sum(r[0] for r in d) / 1e6 / kt_blocks:.3f} ms ({kt_blocks} blocks)")
+        print(f"\ntraffic_bytes_per_block = {fb + wb:.6e}")
+        print(f"fetch_bytes_per_block = {fb:.6e}")
+        print(f"write_bytes_per_block = {wb:.6e}")
+        return
     print(f"\ntraffic_bytes_per_launch = {fetch_b + write_b:.6e}")
     print(f"fetch_bytes_per_launch = {fetch_b:.6e}")
     print(f"write_bytes_per_launch = {write_b:.6e}")
@@ -212,6 +261,14 @@ def main() -> None:
     z.add_argument("--dir", required=True)
     z.add_argument("--workload", default="cfg2")
     z.add_argument("--kernel", default="dedisperse_kernel")
+    z.add_argument("--blocks", type=int, default=0,
+                   help="served run: blocks of each counter run (traffic per block)")
+    z.add_argument("--kt-blocks", type=int, default=0,
+                   help="served run: blocks of the kernel-trace run")
+    sv = sub.add_parser("serve")
+    sv.add_argument("--steps", type=int, default=4)
+    sv.add_argument("--warmup", type=int, default=1)
+    sub.add_parser("lds-probe")
     kt = sub.add_parser("kernels")
     kt.add_argument("--dir", required=True)
     kt.add_argument("--title", default="")
@@ -221,6 +278,10 @@ def main() -> None:
         _steps(args.workload, args.steps, args.tile_group)
     elif args.cmd == "copy":
         _copy()
+    elif args.cmd == "serve":
+        _serve(args.steps, args.warmup)
+    elif args.cmd == "lds-probe":
+        _lds_probe()
     elif args.cmd == "kernels":
         kernel_table(args)
     else:
