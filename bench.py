@@ -447,7 +447,7 @@ def stages_bench(M, args):
                                   d_mean, d_sig),
           D5 * N5 * 4, "config-5 output 2048 x 65536 f32: row statistics + boxcar w = 1..64, S/N > 6")
     dmv = M.DeviceBuffer.from_host(np.linspace(0.0, 1000.0, D5, dtype=np.float32))
-    d_st = M.DeviceBuffer((2 + 2 * D5) * 8)
+    d_st = M.DeviceBuffer(M.median5_stats_len(D5, N5) * 8)
     timed("median5", "median5",
           lambda: M.detect_median5(d_y, D5, N5, N5, dmv, 6.0, d_c, cap, d_n, d_st, per_row=True),
           D5 * N5 * 4, "config-5 output: median-of-five detector, per-row statistics")

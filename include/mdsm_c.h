@@ -411,10 +411,17 @@ int mdsm_detect_boxcar_stream(const float *d_x, uint32_t ndms, uint32_t n, int64
  *   chunk arrays given  : mean = d_chunk_mean[l/spc], rms = d_chunk_rms[l/spc],
  *                         snr = (m-mean)/rms   (process_brute with noise arrays);
  *   per_row_stats != 0  : per-row mean/rms of the n samples (process_subband);
- *   otherwise           : mean/rms over the whole block, FP64 narrowed to float
- *                         (process_brute without noise arrays);  snr = m/rms.
- * d_stats: device scratch of 2+2*ndms doubles (d_stats[0..1] = block mean and
- * sigma in the whole-block mode).  *d_count is zeroed by the call. */
+ *   otherwise           : the whole-block statistics of process_brute without
+ *                         noise arrays (:143-167): the ndms x n values, rows one
+ *                         after another, in chunks of min(values, 32768); each
+ *                         chunk summed in double in index order, the chunk
+ *                         means added to a float mean and the mean squares to a
+ *                         double; the last whole chunk and any partial one are
+ *                         not read; snr = m/rms.
+ * d_stats: device scratch of mdsm_median5_stats_len(ndms, n) doubles
+ * (d_stats[0..1] = block mean and rms in the whole-block mode).  *d_count is
+ * zeroed by the call. */
+uint64_t mdsm_median5_stats_len(uint32_t ndms, uint32_t n);
 int mdsm_detect_median5(const float *d_x, uint32_t ndms, uint32_t n,
                         int64_t stride, const float *d_dm_values, double thr,
                         int32_t per_row_stats, const float *d_chunk_mean,

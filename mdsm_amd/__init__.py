@@ -226,6 +226,8 @@ def lib():
         getattr(L, name).restype = None
     L.mdsm_noalloc_max_work.restype = C.c_uint64
     L.mdsm_noalloc_max_work.argtypes = []
+    L.mdsm_median5_stats_len.restype = C.c_uint64
+    L.mdsm_median5_stats_len.argtypes = [u32, u32]
     L.mdsm_boxcar_carry_len.restype = C.c_uint32
     L.mdsm_boxcar_carry_len.argtypes = [u32]
     L.mdsm_last_error.restype = C.c_char_p
@@ -711,6 +713,11 @@ def detect_boxcar_stream(d_x, ndms, n, stride, d_carry, carry_stride, carry_vali
                                           1 if carry_valid else 0, w, len(widths), thr,
                                           g(d_cands), cap, g(d_count), g(d_mean), g(d_sigma),
                                           _sh(stream)))
+
+
+def median5_stats_len(ndms: int, n: int) -> int:
+    """Doubles of device scratch detect_median5 needs for ndms rows of n samples."""
+    return int(lib().mdsm_median5_stats_len(int(ndms), int(n)))
 
 
 def detect_median5(d_x, ndms, n, stride, d_dmv, thr, d_cands, cap, d_count, d_stats,

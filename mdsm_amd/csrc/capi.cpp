@@ -231,6 +231,13 @@ extern "C" int mdsm_detect_boxcar_stream(const float *d_x, uint32_t ndms, uint32
     return MDSM_OK;
 }
 
+extern "C" uint64_t mdsm_median5_stats_len(uint32_t ndms, uint32_t n)
+{
+    uint64_t mf = 0;
+    const uint64_t chunks = brute_stat_chunks(ndms, n, &mf);
+    return 2 + 2 * std::max<uint64_t>(ndms, chunks);
+}
+
 extern "C" int mdsm_detect_median5(const float *d_x, uint32_t ndms, uint32_t n, int64_t stride,
                                    const float *d_dm_values, double thr, int32_t per_row_stats,
                                    const float *d_chunk_mean, const float *d_chunk_rms,

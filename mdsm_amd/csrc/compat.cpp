@@ -580,7 +580,9 @@ extern "C" float *initialiseMDSM(SURVEY *sv)
               hipok(hipMalloc((void **)&c->d_tail, (size_t)std::max(1u, M) * c->C * 4), "tail") &&
               hipok(hipMalloc((void **)&c->d_out, pi.out_elems * 4), "device output") &&
               hipok(hipMalloc((void **)&c->d_dmv, pi.ndms_total * 4), "dm values") &&
-              hipok(hipMalloc((void **)&c->d_stats, (2 + 2 * (size_t)pi.ndms_total) * 8), "stats") &&
+              hipok(hipMalloc((void **)&c->d_stats,
+                                  mdsm_median5_stats_len(pi.ndms_total, c->nsamp) * 8),
+                        "stats") &&
               hipok(hipMalloc((void **)&c->d_cands, (size_t)nr * kCandCap * sizeof(mdsm_cand)),
                     "candidates") &&
               hipok(hipMalloc((void **)&c->d_count, nr * 4), "count") &&

@@ -1628,28 +1628,66 @@ __device__ __forceinline__ float med5(float a, float b, float c, float d, float 
     return c;
 }
 
-__global__ void block_stats_finish(const double *s1, const double *s2, uint32_t ndms, uint32_t n,
-                                   double *stats)
+// Whole-block statistics of the reference's brute-force output thread
+// without noise arrays (src/mdsm/dedispersion_output.cpp:143-167).  The
+// ndms x n values, rows read one after another as one series, are taken in
+// chunks of mf = min(vals, 32768) values; chunk i's sum and sum of squares
+// are formed in double in index order (one thread per chunk, so the
+// rounding of every addition is the reference's); the loop stops before the
+// chunk that would reach the end of the values, so iters = max(1,
+// ceil(vals / mf) - 1) chunks are read.
+uint32_t brute_stat_chunks(uint32_t ndms, uint32_t n, uint64_t *mf)
 {
-    // single wave: ordered partial sums over rows, then a fixed tree
-    const int lane = threadIdx.x;
-    double a = 0.0, b = 0.0;
-    for (uint32_t d = lane; d < ndms; d += 64) {
-        a += s1[d];
-        b += s2[d];
+    const uint64_t vals = (uint64_t)ndms * n;
+    *mf = vals < 32768 ? vals : 32768;
+    if (*mf == 0) return 0;
+    const uint64_t c = (vals + *mf - 1) / *mf;
+    return (uint32_t)(c > 1 ? c - 1 : 1);
+}
+
+__global__ void __launch_bounds__(256)
+    brute_chunk_sums_kernel(const float *x, uint32_t n, int64_t stride, uint64_t mf,
+                            uint32_t iters, double *tot, double *tot2)
+{
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= iters) return;
+    const uint64_t f = (uint64_t)i * mf;
+    const uint64_t row = f / n;
+    uint32_t col = (uint32_t)(f - row * n);
+    const float *p = x + (int64_t)row * stride;
+    double t = 0.0, t2 = 0.0;
+    for (uint64_t j = 0; j < mf; ++j) {
+        const double v = (double)p[col];
+        t += v;
+        t2 += v * v;  // the square of a float is exact in double, as pow(x, 2) is
+        if (++col == n) {
+            col = 0;
+            p += stride;
+        }
     }
-    a = wave_sum_f64(a);
-    b = wave_sum_f64(b);
-    if (lane == 0) {
-        const double cnt = (double)ndms * (double)n;
-        const double m = a / cnt;
-        const double ex2 = b / cnt;
-        const double mm = m * m;
-        double var = ex2 - mm;
-        if (var < 0.0) var = 0.0;
-        stats[0] = m;
-        stats[1] = sqrt(var);
+    tot[i] = t;
+    tot2[i] = t2;
+}
+
+// The reference's combination, in its arithmetic: the float running mean
+// gains each chunk mean (the addition formed in double, then rounded to
+// float), the double mean square each chunk's; mean = float(sum) / iters in
+// float; rms = float(sqrt(msq / iters - float(mean * mean))).
+__global__ void brute_stats_finish(const double *tot, const double *tot2, uint32_t iters,
+                                   uint64_t mf, double *stats)
+{
+    if (threadIdx.x != 0) return;
+    const double mfd = (double)mf;
+    float lm = 0.0f;
+    double m2 = 0.0;
+    for (uint32_t i = 0; i < iters; ++i) {
+        lm = (float)((double)lm + tot[i] / mfd);
+        m2 += tot2[i] / mfd;
     }
+    lm = lm / (float)iters;
+    const float lm2 = lm * lm;
+    stats[0] = (double)lm;
+    stats[1] = (double)(float)sqrt(m2 / (double)iters - (double)lm2);
 }
 
 // mode 0: statistics over the whole block (stats[0..1]);
@@ -1718,12 +1756,18 @@ hipError_t launch_median5(const float *x, uint32_t ndms, uint32_t n, int64_t str
 {
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, count, 1u, 0u);
     if (ndms == 0 || n < 5) return hipGetLastError();
-    // stats[2 .. 2+2*ndms) holds the per-row sums
-    double *s1 = stats + 2, *s2 = stats + 2 + ndms;
-    if (mode != 2) {
+    // stats[2 .. 2+2*ndms) holds the per-row sums (mode 1), stats[2 ..
+    // 2+2*chunks) the chunk sums (mode 0)
+    if (mode == 1) {
+        double *s1 = stats + 2, *s2 = stats + 2 + ndms;
         hipLaunchKernelGGL(row_sums_kernel, dim3(ndms), dim3(256), 0, s, x, n, stride, s1, s2);
-        if (mode == 0)
-            hipLaunchKernelGGL(block_stats_finish, dim3(1), dim3(64), 0, s, s1, s2, ndms, n, stats);
+    } else if (mode == 0) {
+        uint64_t mf = 0;
+        const uint32_t iters = brute_stat_chunks(ndms, n, &mf);
+        double *t1 = stats + 2, *t2 = stats + 2 + iters;
+        hipLaunchKernelGGL(brute_chunk_sums_kernel, dim3((iters + 255) / 256), dim3(256), 0, s, x, n,
+                           stride, mf, iters, t1, t2);
+        hipLaunchKernelGGL(brute_stats_finish, dim3(1), dim3(64), 0, s, t1, t2, iters, mf, stats);
     }
     dim3 grid(((n - 4 + 3) / 4 + 255) / 256, ndms);
     hipLaunchKernelGGL(median5_kernel, grid, dim3(256), 0, s, x, n, stride, dmv, (float)thr, mode,

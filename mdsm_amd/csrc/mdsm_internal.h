@@ -289,6 +289,8 @@ hipError_t launch_boxcar(const float *x, uint32_t ndms, uint32_t n, int64_t stri
                          uint32_t wmask, double thr, const SqrtW &sqrtw, mdsm_cand *cands,
                          uint32_t cap, uint32_t *count, double *mean, double *sigma,
                          hipStream_t s);
+// Chunks the brute-force block statistics read (mf: values per chunk).
+uint32_t brute_stat_chunks(uint32_t ndms, uint32_t n, uint64_t *mf);
 hipError_t launch_median5(const float *x, uint32_t ndms, uint32_t n, int64_t stride,
                           const float *dmv, double thr, int mode, const float *cmean,
                           const float *crms, uint32_t spc, mdsm_cand *cands, uint32_t cap,

@@ -72,6 +72,8 @@ def lib():
         L.orc_boxcar_stream.argtypes = [vp, i32, i32, i32, i64, vp, i32, f64, vp, i32]
         L.orc_median_of_five.restype = f32
         L.orc_median_of_five.argtypes = [f32] * 5
+        L.orc_brute_stats.restype = None
+        L.orc_brute_stats.argtypes = [vp, i32, i32, i64, C.POINTER(f32), C.POINTER(f32)]
         L.orc_corner_turn_f32.restype = None
         L.orc_corner_turn_f32.argtypes = [vp, i32, i32, i64, i32, vp, i64]
         L.orc_unpack_bits.restype = None
@@ -266,6 +268,38 @@ def median_of_five(a, b, c, d, e):
     return lib().orc_median_of_five(a, b, c, d, e)
 
 
+def brute_stats(x):
+    """orc_brute_stats: the reference output thread's chunked whole-block
+    mean and rms of rows x [ndms][n] (float32 values)."""
+    x = np.ascontiguousarray(x, np.float32)
+    m, r = C.c_float(), C.c_float()
+    lib().orc_brute_stats(_p(x), x.shape[0], x.shape[1], x.shape[1], C.byref(m), C.byref(r))
+    return np.float32(m.value), np.float32(r.value)
+
+
+def process_brute_lines(x, lowdm, dmstep, thr, timestamp, block_rate):
+    """Candidate lines of the reference's brute-force output thread without
+    noise arrays (src/mdsm/dedispersion_output.cpp:143-195) on rows x
+    [tdms][nsamp]: statistics from brute_stats, the median of each 5-sample
+    window (l < nsamp - 4), a line "%lf, %f, %f" (time, DM, median/rms)
+    where median - mean >= rms * thr and the row's DM > 1, all in float."""
+    x = np.ascontiguousarray(x, np.float32)
+    lm, lr = brute_stats(x)
+    thr = np.float32(thr)
+    out = []
+    for k in range(x.shape[0]):
+        dm = np.float32(np.float32(lowdm) + np.float32(0)) + np.float32(k) * np.float32(dmstep)
+        if not dm > 1.0:
+            continue
+        w = np.lib.stride_tricks.sliding_window_view(x[k], 5)
+        med = np.sort(w, axis=1)[:, 2]
+        hit = np.nonzero((med - lm) >= lr * thr)[0]
+        for l in hit:
+            out.append("%f, %f, %f" % (timestamp + float(l) * block_rate, float(dm),
+                                       float(np.float32(med[l] / lr))))
+    return out
+
+
 def corner_turn_f32(x_tm, invert=False):
     """Time-major [nsamp][nchans] -> channel-major [nchans][nsamp] float32."""
     x = np.ascontiguousarray(x_tm, np.float32)
@@ -453,8 +487,29 @@ def ref():
         med5.restype = C.c_float
         med5.argtypes = [C.c_float] * 5
         R.median_of_five = med5
+        if hasattr(R, "mdsm_ref_process_brute"):
+            R.mdsm_ref_process_brute.restype = C.c_int
+            R.mdsm_ref_process_brute.argtypes = [C.c_void_p, C.c_uint, C.c_uint, C.c_int,
+                                                 C.c_float, C.c_float, C.c_float, C.c_double,
+                                                 C.c_double, C.c_char_p]
         _ref = R
     return _ref
+
+
+def ref_process_brute(buf, read_nsamp, lowdm, dmstep, thr, timestamp, block_rate, path):
+    """The reference's own process_brute (oracle/_ref) on a [tdms][nsamp]
+    float buffer, one thread, no noise arrays; its candidate lines go to
+    `path`.  Returns False when the reference build is absent."""
+    R = ref()
+    if R is None or not hasattr(R, "mdsm_ref_process_brute"):
+        return False
+    b = np.ascontiguousarray(buf, np.float32).copy()
+    tdms, nsamp = b.shape
+    rc = R.mdsm_ref_process_brute(_p(b), nsamp, tdms, int(read_nsamp), lowdm, dmstep, thr,
+                                  timestamp, block_rate, str(path).encode())
+    if rc != 0:
+        raise RuntimeError("reference process_brute could not open its output file")
+    return True
 
 
 def ref_cpu_dedisp(fch1, foff, tsamp, nchans, nsamp, startdm, dmstep, tdms,

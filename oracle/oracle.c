@@ -599,6 +599,44 @@ int orc_boxcar_stream(const float *x, int ndms, int n, int nblk, int64_t stride,
 /* Reference in-path detector                                          */
 /* ------------------------------------------------------------------ */
 
+/* Block statistics of the reference's brute-force output thread when no
+ * noise arrays are given (src/mdsm/dedispersion_output.cpp:143-167): the
+ * ndms x n values, rows read one after another as one series, are taken in
+ * chunks of mf = min(vals, 32768); each chunk's sum and sum of squares are
+ * accumulated in double in index order; the chunk means are added to a
+ * float running mean (each addition formed in double, then rounded to
+ * float) and the mean squares to a double; the loop stops before the chunk
+ * that would reach the end of the values, so the last whole chunk and any
+ * partial one are never read (iters = max(1, ceil(vals/mf) - 1)).  mean =
+ * float(mean_sum / iters), rms = float(sqrt(msq_sum/iters - mean*mean)) with
+ * mean*mean a float product. */
+void orc_brute_stats(const float *x, int ndms, int n, int64_t stride, float *mean, float *rms)
+{
+    const uint64_t vals = (uint64_t)ndms * (uint64_t)n;
+    const uint64_t mf = vals < 32768 ? vals : 32768;
+    float localmean = 0.0f;
+    double mean2 = 0.0;
+    unsigned iters = 0;
+    for (;;) {
+        double total = 0.0, total2 = 0.0;
+        for (uint64_t j = 0; j < mf; j++) {
+            const uint64_t f = (uint64_t)iters * mf + j;
+            const double v = (double)x[(f / (uint64_t)n) * (uint64_t)stride + f % (uint64_t)n];
+            total += v;
+            total2 += v * v;
+        }
+        localmean = (float)((double)localmean + total / (double)mf);
+        mean2 += total2 / (double)mf;
+        iters++;
+        if ((uint64_t)iters * mf + mf >= vals)
+            break;
+    }
+    localmean = localmean / (float)iters;
+    const float lm2 = localmean * localmean;
+    *mean = localmean;
+    *rms = (float)sqrt(mean2 / (double)iters - (double)lm2);
+}
+
 /* Median of five values (src/mdsm/dedispersion_output.cpp:14-53), stated
  * here as an insertion sort and the middle element. */
 float orc_median_of_five(float a, float b, float c, float d, float e)

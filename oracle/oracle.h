@@ -41,6 +41,10 @@
  *   orc_median_of_five
  *       all 5^5 tuples against the reference's own medianOfFive built under
  *       oracle/_ref and against sort.
+ *   orc_brute_stats
+ *       the candidate file of the reference's own process_brute (oracle/_ref)
+ *       on multi-chunk blocks, line for line, and a committed golden vector
+ *       of its statistics.
  *   orc_corner_turn_f32
  *       numpy transpose (and channel reversal) of the same block.
  *   orc_detrend_f64
@@ -142,6 +146,9 @@ int orc_boxcar_stream(const float *x, int ndms, int n, int nblk, int64_t stride,
 
 /* --- reference in-path detector (median of five) --------------------- */
 float orc_median_of_five(float a, float b, float c, float d, float e);
+/* Whole-block statistics of the brute-force output thread without noise
+ * arrays (chunks of 32768 values; src/mdsm/dedispersion_output.cpp:143-167). */
+void orc_brute_stats(const float *x, int ndms, int n, int64_t stride, float *mean, float *rms);
 
 /* Corner turn of a time-major float block (MdsmModule.cpp:135-151). */
 void orc_corner_turn_f32(const float *in, int nsamp, int nchans, int64_t in_stride,

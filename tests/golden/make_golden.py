@@ -13,6 +13,10 @@ from the reference sources (make -C oracle ref).  The fixtures are data only
                            table and its dedispersed time-major output.
   ref_median5.npz          medianOfFive (src/mdsm/dedispersion_output.cpp:14-53)
                            on every tuple of {0..4}^5 and on random tuples.
+  ref_process_brute.json   the candidate lines of the reference's process_brute
+                           (dedispersion_output.cpp:135-213, one thread, no noise
+                           arrays) on seeded [tdms][nsamp] blocks: fewer values
+                           than one chunk, whole chunks, a partial last chunk.
 
     python tests/golden/make_golden.py
 """
@@ -36,6 +40,18 @@ SHIFT_CASES = [
     ("cfg3_brute_equiv", 1500.0, -400.0 / 4096, 54.6e-6, 4096, 0.0, 1500.0 / 4096, 4096),
     ("cfg5", 1500.0, -300.0 / 1024, 64e-6, 1024, 0.0, 1000.0 / 2048, 2048),
 ]
+
+
+# (seed, tdms, nsamp, threshold): 3000 values (< one chunk of 32768), 65536
+# (two whole chunks: the second is never read), 200000 (6 of 7 chunks read)
+PROCESS_BRUTE_CASES = [(31, 3, 1000, 3.0), (32, 16, 4096, 2.5), (33, 40, 5000, 3.0)]
+
+
+def process_brute_input(seed, tdms, nsamp):
+    rng = np.random.default_rng(seed)
+    x = (100.0 + 10.0 * rng.standard_normal((tdms, nsamp))).astype(np.float32)
+    x[tdms // 2, nsamp // 3:nsamp // 3 + 12] += 150.0
+    return x
 
 
 def digest(tab):
@@ -81,6 +97,22 @@ def main():
     tup = np.concatenate([grid, rnd])
     med = np.array([R.median_of_five(*map(float, t)) for t in tup], np.float32)
     np.savez_compressed(os.path.join(HERE, "ref_median5.npz"), tuples=tup, median=med)
+
+    # process_brute: inputs regenerated from the seed, 100 + 10 N(0,1) float32
+    import tempfile
+    cases = []
+    for seed, tdms, nsamp, thr in PROCESS_BRUTE_CASES:
+        x = process_brute_input(seed, tdms, nsamp)
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "cands.dat")
+            oracle.ref_process_brute(x, nsamp, 1.5, 0.25, thr, 1234.5, 64e-6, path)
+            with open(path) as f:
+                lines = f.read().strip().splitlines()
+        cases.append({"seed": seed, "tdms": tdms, "nsamp": nsamp, "thr": thr, "lowdm": 1.5,
+                      "dmstep": 0.25, "timestamp": 1234.5, "block_rate": 64e-6, "lines": lines})
+        print("process_brute", tdms, nsamp, len(lines), "lines")
+    with open(os.path.join(HERE, "ref_process_brute.json"), "w") as f:
+        json.dump(cases, f, indent=0)
     print("fixtures written")
 
 

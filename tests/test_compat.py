@@ -139,5 +139,19 @@ def test_manager_pipeline_matches_oracle(gpu, tmp_path):
     ref = oracle.dedisperse_f32(stream, sh, nsamp * nblocks)
     for b in range(nblocks):
         assert np.array_equal(got[b], ref[:, b * nsamp:(b + 1) * nsamp]), f"block {b}"
+    # the candidate file is the reference output thread's: process_brute
+    # without noise arrays (chunked block statistics) on every block, line
+    # for line -- against the reference's own process_brute where
+    # oracle/_ref is built, and against the oracle's restatement of it
     txt = (tmp_path / "cands.dat").read_text().strip().splitlines()
-    assert len(txt) > 0 and all(len(line.split(",")) == 3 for line in txt)
+    exp = []
+    for b in range(nblocks):
+        exp += oracle.process_brute_lines(got[b], 0.0, dmstep, 5.0, 0.0, 64e-6)
+    assert txt == exp and len(txt) > 0
+    if oracle.ref() is not None:
+        ref = []
+        for b in range(nblocks):
+            path = tmp_path / f"ref{b}.dat"
+            assert oracle.ref_process_brute(got[b], nsamp, 0.0, dmstep, 5.0, 0.0, 64e-6, path)
+            ref += path.read_text().strip().splitlines()
+        assert txt == ref

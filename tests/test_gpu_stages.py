@@ -160,9 +160,11 @@ def test_boxcar_capacity_overflow_reported(gpu):
 
 # ---------------------------------------------------------------- median of five
 def _np_median5(x, dmv, thr):
-    mean = float(x.astype(np.float64).sum()) / x.size
-    var = float((x.astype(np.float64) ** 2).sum()) / x.size - mean * mean
-    lm, lr = np.float32(mean), np.float32(np.sqrt(var))
+    """Whole-block statistics by the reference output thread's chunked rule
+    (oracle.brute_stats, pinned to the reference's process_brute), then the
+    median-of-five threshold in float."""
+    import oracle
+    lm, lr = oracle.brute_stats(x)
     res = []
     for d in range(x.shape[0]):
         if not dmv[d] > 1.0:
@@ -192,7 +194,8 @@ def test_median5_detector(gpu, n, stride):
     dx = M.DeviceBuffer.from_host(xs)
     ddm = M.DeviceBuffer.from_host(dmv)
     cap = 4096
-    dc, dn, dst = M.DeviceBuffer(cap * 16), M.DeviceBuffer(4), M.DeviceBuffer((2 + 2 * D) * 8)
+    dc, dn = M.DeviceBuffer(cap * 16), M.DeviceBuffer(4)
+    dst = M.DeviceBuffer(M.median5_stats_len(D, n) * 8)      # 4 chunks of 32768: 3 read
     M.detect_median5(dx, D, n, stride, ddm, 3.0, dc, cap, dn, dst)
     M.synchronize()
     got = M.read_candidates(dc, dn, cap)

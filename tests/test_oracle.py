@@ -410,6 +410,56 @@ def test_boxcar_stream_straddling_pulse_found_once():
         assert n - 8 not in starts
 
 
+# ---------------------------------------------------------------- process_brute statistics
+def _process_brute_input(seed, tdms, nsamp):
+    rng = np.random.default_rng(seed)
+    x = (100.0 + 10.0 * rng.standard_normal((tdms, nsamp))).astype(np.float32)
+    x[tdms // 2, nsamp // 3:nsamp // 3 + 12] += 150.0
+    return x
+
+
+def test_process_brute_matches_reference_golden():
+    """The oracle's brute-force output-thread detector (chunked block
+    statistics, median of five, float thresholds) reproduces the reference
+    process_brute's candidate file line for line: a block smaller than one
+    chunk, two whole chunks (the second never read) and a partial chunk."""
+    with open(os.path.join(GOLD, "ref_process_brute.json")) as f:
+        cases = json.load(f)
+    for c in cases:
+        x = _process_brute_input(c["seed"], c["tdms"], c["nsamp"])
+        got = oracle.process_brute_lines(x, c["lowdm"], c["dmstep"], c["thr"], c["timestamp"],
+                                         c["block_rate"])
+        assert got == c["lines"] and len(got) > 0, (c["tdms"], c["nsamp"])
+
+
+def test_brute_stats_skip_the_last_chunk():
+    """Closed forms of the chunk rule: with two whole chunks only the first is
+    read; with fewer values than a chunk all are read."""
+    x = np.zeros((2, 32768), np.float32)
+    x[0] = 3.0
+    x[1] = 100.0                     # the second chunk: never read
+    m, r = oracle.brute_stats(x)
+    assert m == np.float32(3.0) and r == np.float32(0.0)
+    y = np.array([[1, 2, 3, 4, 5, 6]], np.float32)
+    m, r = oracle.brute_stats(y)
+    assert m == np.float32(3.5) and abs(float(r) - np.sqrt(35.0 / 12.0)) < 1e-6
+
+
+def test_process_brute_matches_reference_build(tmp_path):
+    """Against the reference's own process_brute built under oracle/_ref, on
+    blocks not in the golden file (skipped where that build is absent)."""
+    if oracle.ref() is None:
+        pytest.skip("oracle/_ref not built")
+    rng = np.random.default_rng(77)
+    for tdms, nsamp in ((7, 9000), (33, 3000), (128, 1024)):
+        x = (50.0 + 5.0 * rng.standard_normal((tdms, nsamp))).astype(np.float32)
+        x[5, 10:30] += 60.0          # row 5: DM 2.5 > 1
+        path = tmp_path / f"c{tdms}.dat"
+        assert oracle.ref_process_brute(x, nsamp, 0.0, 0.5, 2.0, 10.0, 1e-3, path)
+        ref = path.read_text().strip().splitlines()
+        assert oracle.process_brute_lines(x, 0.0, 0.5, 2.0, 10.0, 1e-3) == ref and ref
+
+
 # ---------------------------------------------------------------- median of five
 def test_median_of_five_matches_reference_golden():
     g = np.load(os.path.join(GOLD, "ref_median5.npz"))
