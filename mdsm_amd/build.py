@@ -53,7 +53,28 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print("built", LIB)
+    build_driver(verbose)
     return LIB
+
+
+DRIVER_SRC = os.path.join(HERE, "driver", "mdsm_file.cpp")
+DRIVER = os.path.join(HERE, "mdsm_file")
+
+
+def build_driver(verbose: bool = False) -> str:
+    """The standalone file pipeline (driver/mdsm_file.cpp), a plain C++
+    program linked against libmdsm_hip.so (found next to it at run time)."""
+    cmd = [os.environ.get("CXX", "g++"), "-O2", "-std=c++17", "-Wall",
+           "-I" + os.path.join(HERE, "..", "include"), DRIVER_SRC, "-o", DRIVER + ".tmp",
+           "-L" + os.path.dirname(LIB), "-l:" + os.path.basename(LIB),
+           "-Wl,-rpath,$ORIGIN"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"driver build failed:\n{r.stdout}\n{r.stderr}")
+    os.replace(DRIVER + ".tmp", DRIVER)
+    if verbose:
+        print("built", DRIVER)
+    return DRIVER
 
 
 if __name__ == "__main__":
