@@ -466,22 +466,12 @@ def batch_bench(M, w, args):
     samples uniform in [2^16, 2^19], uint8 or float16, 1-3 DM ranges with
     256-4096 trials in total and decimation 1-8 -- run as one batch: the
     autotuner picks the launch shape for the batch, then steady-state steps
-    replay a captured HIP graph.  Inputs are windows of two resident pools
-    (one uint8, one float16, 4096 x 2^19 each, seeded noise with pulses)."""
+    replay a captured HIP graph.  Every beam has its own input block in HBM,
+    made on the device by the seeded generator (uint8: N(128, 16); float16:
+    N(0, 1) with per-channel gains in [0.5, 2]) with 0-4 seeded pulses
+    inside its plan's DM span."""
     rng = np.random.default_rng(w["seed"])
     fch1, bw, tsamp = 1500.0, 300.0, 64e-6
-    Cmax, Smax = 4096, 1 << 19
-    print("[bench] cfg4: generating the input pools", file=sys.stderr, flush=True)
-    pools = {
-        "u8": M.synth_filterbank(Cmax, Smax, "u8", seed=401, fch1=fch1, foff=-bw / Cmax,
-                                 tsamp=tsamp, mean=128.0, sigma=16.0,
-                                 pulses=[(300.0, 200_000, 8, 1.0)], nthreads=16),
-        "f16": M.synth_filterbank(Cmax, Smax, "f16", seed=402, fch1=fch1, foff=-bw / Cmax,
-                                  tsamp=tsamp, gain=(0.5, 2.0),
-                                  pulses=[(300.0, 200_000, 8, 0.5)], nthreads=16),
-    }
-    d_pool = {k: M.DeviceBuffer.from_host(v) for k, v in pools.items()}
-    del pools
     plans, beams = [], []
     while len(plans) < w["beams"]:
         C = int(rng.choice([256, 1024, 4096]))
@@ -506,9 +496,26 @@ def batch_bench(M, w, args):
     units = sum(p.info.dm_chan_samples for p in plans)
     out_bytes = sum(p.info.out_elems * 4 for p in plans)
     print(f"[bench] cfg4: {len(plans)} beams, {units:.3e} DM*chan*samples per step, "
-          f"outputs {out_bytes / 1e9:.1f} GB", file=sys.stderr, flush=True)
-    ins = [d_pool[b["dtype"]] for b in beams]
-    strides = [Smax] * len(beams)
+          f"outputs {out_bytes / 1e9:.1f} GB; generating the inputs on the device",
+          file=sys.stderr, flush=True)
+    prng = np.random.default_rng(w["seed"] + 1)
+    ins, npulses = [], 0
+    for i, (b, p) in enumerate(zip(beams, plans)):
+        C, S, dt = b["C"], b["S"], b["dtype"]
+        hi = b["ranges"][-1][0] + b["ranges"][-1][1] * b["ranges"][-1][2]
+        pulses = [(float(prng.uniform(0.1 * hi, 0.9 * hi)),
+                   int(prng.integers(0, max(1, S - p.info.maxshift))),
+                   int(prng.integers(1, 33)), 1.0 if dt == "u8" else 0.5)
+                  for _ in range(int(prng.integers(0, 5)))]
+        npulses += len(pulses)
+        d = M.DeviceBuffer(C * S * (1 if dt == "u8" else 2))
+        extra = dict(mean=128.0, sigma=16.0) if dt == "u8" else dict(gain=(0.5, 2.0))
+        M.synth_filterbank_device(d, C, S, dt, seed=4000 + i, fch1=fch1, foff=-bw / C,
+                                  tsamp=tsamp, pulses=pulses, **extra)
+        ins.append(d)
+    M.synchronize()
+    in_bytes = sum(d.nbytes for d in ins)
+    strides = [b["S"] for b in beams]
     outs = [M.DeviceBuffer(p.info.out_elems * 4) for p in plans]
     batch = M.Batch(plans)
     s = M.Stream()
@@ -537,11 +544,13 @@ def batch_bench(M, w, args):
         "value": units * args.steps / wall, "unit": "DM*chan*samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": wall * 1e3 / args.steps,
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
-        "dtype": "u8/f16->f32", "data": "synthetic (seeded pools with pulses)",
+        "dtype": "u8/f16->f32", "data": "synthetic (seeded, generated on the device, per beam)",
         "config": {"workload": f"cfg4: {len(plans)} beams (uint8 {n_u8}, float16 "
                    f"{len(plans) - n_u8}), C in {{256,1024,4096}}, S in [2^16,2^19], "
-                   "1-3 ranges, 256-4096 trials, k 1-8", "seed": w["seed"],
-                   "dm_chan_samples_per_step": units, "output_bytes": out_bytes},
+                   "1-3 ranges, 256-4096 trials, k 1-8; a distinct input block per beam "
+                   f"with 0-4 pulses ({npulses} in all)", "seed": w["seed"],
+                   "dm_chan_samples_per_step": units, "input_bytes": in_bytes,
+                   "output_bytes": out_bytes},
         "tuning": tuning, "autotune_seconds": tune_s,
         "eager_step_ms": eager_s * 1e3, "graph_step_ms": wall * 1e3 / args.steps,
     }

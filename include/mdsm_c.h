@@ -669,6 +669,15 @@ typedef struct {
 int mdsm_synth_filterbank(const mdsm_synth_spec *spec, void *host_out,
                           int64_t out_stride);
 
+/* The same generator on the device: d_out is [nchans][out_stride] of dtype
+ * in device memory, written on `stream` (at most 16 pulses, nchans <= 65535;
+ * nthreads is ignored).  Agrees with mdsm_synth_filterbank to the rounding of
+ * the device's log and cos (a uint8 sample can differ by one where the value
+ * falls on a rounding boundary).  For benchmark inputs too large to make on
+ * the host. */
+int mdsm_synth_filterbank_device(const mdsm_synth_spec *spec, void *d_out, int64_t out_stride,
+                                 void *stream);
+
 /* ------------------------------------------------------------------ */
 /* Device plumbing                                                     */
 /* ------------------------------------------------------------------ */

@@ -170,6 +170,7 @@ def lib():
         "mdsm_detect_median5": [vp, u32, u32, i64, vp, f64, i32, vp, vp, u32, vp, u32, vp, vp,
                                 vp],
         "mdsm_synth_filterbank": [C.POINTER(SynthSpec), vp, i64],
+        "mdsm_synth_filterbank_device": [C.POINTER(SynthSpec), vp, i64, vp],
         "mdsm_timing_enable": [C.c_int],
         "mdsm_timing_reset": [],
         "mdsm_timing_get": [C.c_char_p, C.POINTER(f64), C.POINTER(u64)],
@@ -741,6 +742,21 @@ def read_candidates(d_cands, d_count, cap) -> np.ndarray:
 
 
 # ---------------------------------------------------------------- synthetic data
+def synth_filterbank_device(d_out, nchans, nsamp, dtype="u8", seed=1, fch1=1500.0,
+                            foff=-0.390625, tsamp=64e-6, mean=0.0, sigma=1.0, gain=(1.0, 1.0),
+                            pulses=(), stride=None, stream=None):
+    """synth_filterbank into the device buffer d_out ([nchans][stride])."""
+    dt = DTYPES[dtype] if isinstance(dtype, str) else int(dtype)
+    stride = nsamp if stride is None else stride
+    P = (Pulse * max(1, len(pulses)))()
+    for i, (dm, t0, w, a) in enumerate(pulses):
+        P[i] = Pulse(float(dm), int(t0), int(w), float(a), 0)
+    spec = SynthSpec(int(seed), dt, nchans, nsamp, fch1, foff, tsamp, mean, sigma, gain[0],
+                     gain[1], len(pulses), P, 0)
+    ptr = d_out.ptr if isinstance(d_out, DeviceBuffer) else d_out
+    check(lib().mdsm_synth_filterbank_device(C.byref(spec), ptr, stride, _sh(stream)))
+
+
 def synth_filterbank(nchans, nsamp, dtype="u8", seed=1, fch1=1500.0, foff=-0.390625,
                      tsamp=64e-6, mean=0.0, sigma=1.0, gain=(1.0, 1.0), pulses=(),
                      stride=None, nthreads=0) -> np.ndarray:
