@@ -446,6 +446,15 @@ def stages_bench(M, args):
           lambda: M.detect_boxcar(d_y, D5, N5, N5, [1, 2, 4, 8, 16, 32, 64], 6.0, d_c, cap, d_n,
                                   d_mean, d_sig),
           D5 * N5 * 4, "config-5 output 2048 x 65536 f32: row statistics + boxcar w = 1..64, S/N > 6")
+    Hc = M.boxcar_carry_len([1, 2, 4, 8, 16, 32, 64])
+    d_carry = M.DeviceBuffer(D5 * Hc * 4)
+    d_carry.fill_u32(0)
+    timed("boxcar_stream_w1_64", "detect",
+          lambda: M.detect_boxcar_stream(d_y, D5, N5, N5, d_carry, Hc, True,
+                                         [1, 2, 4, 8, 16, 32, 64], 6.0, d_c, cap, d_n, d_mean,
+                                         d_sig),
+          D5 * N5 * 4, "config-5 output: streaming search (68 carried samples per row, then the "
+          "block), row statistics + boxcar w = 1..64, S/N > 6")
     dmv = M.DeviceBuffer.from_host(np.linspace(0.0, 1000.0, D5, dtype=np.float32))
     d_st = M.DeviceBuffer(M.median5_stats_len(D5, N5) * 8)
     timed("median5", "median5",
