@@ -298,6 +298,31 @@ __device__ __forceinline__ void fetch8(const void *src, int64_t rowbase, int col
     }
 }
 
+// The four 16-bit phase entries of one entry column from its 8 raw samples:
+// phase ph is (d0[ph], d1[ph]).  Phase ph+1's first dword is phase ph's
+// second (d0[ph+1] == d1[ph]).
+template <int DT>
+__device__ __forceinline__ void phases16(const uint32_t *raw, uint32_t (&d0)[4], uint32_t (&d1)[4])
+{
+    if constexpr (DT == DT_U8) {
+        const uint32_t a = raw[0], b = raw[1];  // samples 0-3, 4-7
+#pragma unroll
+        for (int ph = 0; ph < 4; ++ph) {
+            d0[ph] = perm(b, a, 0x0c000c00u | (uint32_t)ph | ((uint32_t)(ph + 2) << 16));
+            d1[ph] = perm(b, a, 0x0c000c00u | (uint32_t)(ph + 1) | ((uint32_t)(ph + 3) << 16));
+        }
+    } else {
+        const uint32_t h0 = raw[0], h1 = raw[1], h2 = raw[2], h3 = raw[3];
+        const uint32_t P0 = perm(h1, h0, 0x05040100u), Q0 = perm(h1, h0, 0x07060302u);
+        const uint32_t P1 = perm(h2, h1, 0x05040100u), Q1 = perm(h2, h1, 0x07060302u);
+        const uint32_t P2 = perm(h3, h2, 0x05040100u);
+        d0[0] = P0; d1[0] = Q0;
+        d0[1] = Q0; d1[1] = P1;
+        d0[2] = P1; d1[2] = Q1;
+        d0[3] = Q1; d1[3] = P2;
+    }
+}
+
 // Write the four phases of entry column k from its 8 raw samples.
 template <int DT>
 __device__ __forceinline__ void store_phases(char *row, int K, int k, const uint32_t *raw)
@@ -318,23 +343,7 @@ __device__ __forceinline__ void store_phases(char *row, int K, int k, const uint
         return;
     } else {
         uint32_t d0[4], d1[4];
-        if constexpr (DT == DT_U8) {
-            const uint32_t a = raw[0], b = raw[1];  // samples 0-3, 4-7
-#pragma unroll
-            for (int ph = 0; ph < 4; ++ph) {
-                d0[ph] = perm(b, a, 0x0c000c00u | (uint32_t)ph | ((uint32_t)(ph + 2) << 16));
-                d1[ph] = perm(b, a, 0x0c000c00u | (uint32_t)(ph + 1) | ((uint32_t)(ph + 3) << 16));
-            }
-        } else {
-            const uint32_t h0 = raw[0], h1 = raw[1], h2 = raw[2], h3 = raw[3];
-            const uint32_t P0 = perm(h1, h0, 0x05040100u), Q0 = perm(h1, h0, 0x07060302u);
-            const uint32_t P1 = perm(h2, h1, 0x05040100u), Q1 = perm(h2, h1, 0x07060302u);
-            const uint32_t P2 = perm(h3, h2, 0x05040100u);
-            d0[0] = P0; d1[0] = Q0;
-            d0[1] = Q0; d1[1] = P1;
-            d0[2] = P1; d1[2] = Q1;
-            d0[3] = Q1; d1[3] = P2;
-        }
+        phases16<DT>(raw, d0, d1);
 #pragma unroll
         for (int ph = 0; ph < 4; ++ph) ((uint2 *)row)[ph * K + k] = make_uint2(d0[ph], d1[ph]);
     }
@@ -350,10 +359,23 @@ constexpr int pf_iters() { return max_window(DT) / 256; }
 template <int DT>
 using PreRegs = uint32_t[kPfRows][pf_iters<DT>()][Layout<DT>::raw];
 
+// Window start (sample column) of each row a wave stages in a chunk, loaded a
+// chunk ahead of its use: a vector load through a global-address-space view
+// (a scalar load would tie the prefetch to the LDS waits, a flat one would
+// wait for both counters).
+__device__ __forceinline__ void load_colbase(int (&cb)[kPfRows], const int32_t *ab, int cc, int wave)
+{
+#pragma unroll
+    for (int rr = 0; rr < kPfRows; ++rr) {
+        const int i = wave + kWaves * rr;
+        cb[rr] = i < cc ? gptr<int32_t>(ab)[i] : 0;
+    }
+}
+
 template <int DT>
 __device__ __forceinline__ void fetch_chunk(PreRegs<DT> &pre, const void *src, int64_t in_off,
-                                            int64_t row0, int64_t rs, const int32_t *ab, int t0,
-                                            int K, int cc, int in_len, int wave, int lane)
+                                            int64_t row0, int64_t rs, const int (&cb)[kPfRows],
+                                            int t0, int K, int cc, int in_len, int wave, int lane)
 {
     constexpr int es = Layout<DT>::es;
 #pragma unroll
@@ -361,7 +383,7 @@ __device__ __forceinline__ void fetch_chunk(PreRegs<DT> &pre, const void *src, i
         const int i = wave + kWaves * rr;
         if (i >= cc) break;
         const int64_t rowbase = in_off + (row0 + i) * rs;
-        const int col0 = ab[i] + t0;  // a vector load: a scalar one would tie the prefetch to LGKM waits
+        const int col0 = cb[rr] + t0;
         const bool aligned = ((rowbase * es) & 3) == 0;
 #pragma unroll
         for (int it = 0; it < pf_iters<DT>(); ++it) {
@@ -381,11 +403,36 @@ __device__ __forceinline__ void store_chunk(const PreRegs<DT> &pre, char *buf, i
         const int i = wave + kWaves * rr;
         if (i >= cc) break;
         char *row = buf + (size_t)i * rb;
+        if constexpr (DT == DT_F32) {
 #pragma unroll
-        for (int it = 0; it < pf_iters<DT>(); ++it) {
-            const int k = lane + 64 * it;
-            if (64 * it >= K) break;
-            if (k < K) store_phases<DT>(row, K, k, pre[rr][it]);
+            for (int it = 0; it < pf_iters<DT>(); ++it) {
+                const int k = lane + 64 * it;
+                if (64 * it >= K) break;
+                if (k < K) store_phases<DT>(row, K, k, pre[rr][it]);
+            }
+        } else {
+            // the lane's entry in each of the four phase planes (8*K bytes
+            // apart), computed once per row; entry column lane + 64*it sits
+            // 512*it bytes further.  Each entry is written as two dwords, so
+            // the shared dword of neighbouring phases needs no register copy.
+            uint32_t *q[4];
+            q[0] = (uint32_t *)(row + 8 * lane);
+#pragma unroll
+            for (int ph = 1; ph < 4; ++ph) q[ph] = (uint32_t *)((char *)q[ph - 1] + 8 * K);
+#pragma unroll
+            for (int it = 0; it < pf_iters<DT>(); ++it) {
+                const int k = lane + 64 * it;
+                if (64 * it >= K) break;
+                if (k < K) {
+                    uint32_t d0[4], d1[4];
+                    phases16<DT>(pre[rr][it], d0, d1);
+#pragma unroll
+                    for (int ph = 0; ph < 4; ++ph) {
+                        q[ph][128 * it] = d0[ph];
+                        q[ph][128 * it + 1] = d1[ph];
+                    }
+                }
+            }
         }
     }
 }
@@ -623,35 +670,45 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 row(vo[pr], 1);
             }
         } else {
-#pragma unroll
-            for (int pr = 0; pr < 4; ++pr) {
-                const int i = 2 * pr;
-                if (i >= cc) break;
+            // A rolled loop over the chunk's full pairs: the accumulators are
+            // loop-carried and updated in place.  (Unrolled with an exit per
+            // pair count, the exits' merges cost 16 register copies per pair.)
+            // The pair's offset register is vo[0]; the others move down one
+            // place per pair.
+            int v0 = vo[0], v1 = vo[1], v2 = vo[2], v3 = vo[3];
+            const int np = cc >> 1;
+#pragma unroll 1
+            for (int pr = 0; pr < np; ++pr) {
                 // one pair's reads at a time: hoisting the next pair's reads
                 // above these adds would need twice the data registers
                 __builtin_amdgcn_sched_barrier(0);
-                if (i + 1 < cc) {
-                    uint2 x[R], y[R];
+                uint2 x[R], y[R];
 #pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        x[r] = *(const uint2 *)(lane_base + bcast16(vo[pr], r));
-                        y[r] = *(const uint2 *)(lane_base + bcast16(vo[pr], 8 + r));
-                    }
-#pragma unroll
-                    for (int r = 0; r < R; ++r) {
-                        if constexpr (kInt) {
-                            ae[r] = ae[r] + x[r].x + y[r].x;
-                            ao[r] = ao[r] + x[r].y + y[r].y;
-                        } else {
-                            add_entry(r, x[r]);
-                            add_entry(r, y[r]);
-                        }
-                    }
-                } else {
-#pragma unroll
-                    for (int r = 0; r < R; ++r)
-                        add_entry(r, *(const uint2 *)(lane_base + bcast16(vo[pr], r)));
+                for (int r = 0; r < R; ++r) {
+                    x[r] = *(const uint2 *)(lane_base + bcast16(v0, r));
+                    y[r] = *(const uint2 *)(lane_base + bcast16(v0, 8 + r));
                 }
+                // all 16 reads in flight before the first add waits on one
+                __builtin_amdgcn_sched_barrier(0);
+                v0 = v1;
+                v1 = v2;
+                v2 = v3;
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    if constexpr (kInt) {
+                        ae[r] = ae[r] + x[r].x + y[r].x;
+                        ao[r] = ao[r] + x[r].y + y[r].y;
+                    } else {
+                        add_entry(r, x[r]);
+                        add_entry(r, y[r]);
+                    }
+                }
+            }
+            if (cc & 1) {  // the chunk's last row on its own
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    add_entry(r, *(const uint2 *)(lane_base + bcast16(v0, r)));
             }
         }
     };
@@ -666,11 +723,14 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         load_offsets(vo, 0);
     else
         load_soffsets(relb + wave * kRPW);
+    int cb[kPfRows];  // window starts of the rows the next prefetch stages
     __syncthreads();  // the previous item (persistent mode) is done with the buffers
     {
         const int cc0 = min(ccr, bd.nrows);
-        fetch_chunk<DT>(pre, src, bd.in_off, bd.row0, rs, ab, t0, K, cc0, bd.in_len, wave, lane);
+        load_colbase(cb, ab, cc0, wave);
+        fetch_chunk<DT>(pre, src, bd.in_off, bd.row0, rs, cb, t0, K, cc0, bd.in_len, wave, lane);
         store_chunk<DT>(pre, lds, rb, K, cc0, wave, lane);
+        if (nch > 1) load_colbase(cb, ab + ccr, min(ccr, bd.nrows - ccr), wave);
     }
     __syncthreads();
     for (int ch = 0; ch < nch; ++ch) {
@@ -679,8 +739,14 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         const bool more = ch + 1 < nch;
         const int c1 = c0 + ccr;
         const int cc1 = more ? min(ccr, bd.nrows - c1) : 0;
+        // uint8: every vector load issued before this point (this chunk's
+        // offset registers, the window starts) has landed by now; waiting
+        // here, on every path, keeps the trial loop's reads of the offset
+        // registers from waiting for the prefetch below (the counter is in
+        // order).
+        if constexpr (kVOff) __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
         if (more)  // next chunk's global loads overlap this chunk's compute
-            fetch_chunk<DT>(pre, src, bd.in_off, bd.row0 + c1, rs, ab + c1, t0, K, cc1, bd.in_len,
+            fetch_chunk<DT>(pre, src, bd.in_off, bd.row0 + c1, rs, cb, t0, K, cc1, bd.in_len,
                             wave, lane);
         if constexpr (kInt) {
             if (acc_rows + cc > bd.flush) {
@@ -696,6 +762,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         acc_rows += cc;
         if (more) {
             // in flight across the staging stores and the barrier
+            if (ch + 2 < nch) load_colbase(cb, ab + c1 + ccr, min(ccr, bd.nrows - c1 - ccr), wave);
             if constexpr (kVOff) load_offsets(vo, c1);
             store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, rb, K, cc1, wave, lane);
         }
