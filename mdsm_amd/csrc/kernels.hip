@@ -359,16 +359,22 @@ constexpr int pf_iters() { return max_window(DT) / 256; }
 template <int DT>
 using PreRegs = uint32_t[kPfRows][pf_iters<DT>()][Layout<DT>::raw];
 
-// Window start (sample column) of each row a wave stages in a chunk, loaded a
-// chunk ahead of its use: a vector load through a global-address-space view
-// (a scalar load would tie the prefetch to the LDS waits, a flat one would
-// wait for both counters).
+// Window start (sample column) of each row a wave stages in a chunk: a
+// vector load (a scalar one would tie the prefetch to the LDS waits).  The
+// uint8 path loads it a chunk ahead of its use, through a global-address-space
+// view; the other paths load it just before the prefetch through a generic
+// pointer, whose flat load waits for both counters, the scalar offsets of the
+// chunk's first row (pair) included (measured faster for them).
+template <bool kAhead>
 __device__ __forceinline__ void load_colbase(int (&cb)[kPfRows], const int32_t *ab, int cc, int wave)
 {
 #pragma unroll
     for (int rr = 0; rr < kPfRows; ++rr) {
         const int i = wave + kWaves * rr;
-        cb[rr] = i < cc ? gptr<int32_t>(ab)[i] : 0;
+        if constexpr (kAhead)
+            cb[rr] = i < cc ? gptr<int32_t>(ab)[i] : 0;
+        else
+            cb[rr] = i < cc ? ab[i] : 0;
     }
 }
 
@@ -727,10 +733,11 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
     __syncthreads();  // the previous item (persistent mode) is done with the buffers
     {
         const int cc0 = min(ccr, bd.nrows);
-        load_colbase(cb, ab, cc0, wave);
+        load_colbase<kVOff>(cb, ab, cc0, wave);
         fetch_chunk<DT>(pre, src, bd.in_off, bd.row0, rs, cb, t0, K, cc0, bd.in_len, wave, lane);
         store_chunk<DT>(pre, lds, rb, K, cc0, wave, lane);
-        if (nch > 1) load_colbase(cb, ab + ccr, min(ccr, bd.nrows - ccr), wave);
+        if constexpr (kVOff)
+            if (nch > 1) load_colbase<true>(cb, ab + ccr, min(ccr, bd.nrows - ccr), wave);
     }
     __syncthreads();
     for (int ch = 0; ch < nch; ++ch) {
@@ -745,9 +752,11 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         // registers from waiting for the prefetch below (the counter is in
         // order).
         if constexpr (kVOff) __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
-        if (more)  // next chunk's global loads overlap this chunk's compute
+        if (more) {  // next chunk's global loads overlap this chunk's compute
+            if constexpr (!kVOff) load_colbase<false>(cb, ab + c1, cc1, wave);
             fetch_chunk<DT>(pre, src, bd.in_off, bd.row0 + c1, rs, cb, t0, K, cc1, bd.in_len,
                             wave, lane);
+        }
         if constexpr (kInt) {
             if (acc_rows + cc > bd.flush) {
                 flush();
@@ -762,8 +771,11 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         acc_rows += cc;
         if (more) {
             // in flight across the staging stores and the barrier
-            if (ch + 2 < nch) load_colbase(cb, ab + c1 + ccr, min(ccr, bd.nrows - c1 - ccr), wave);
-            if constexpr (kVOff) load_offsets(vo, c1);
+            if constexpr (kVOff) {
+                if (ch + 2 < nch)
+                    load_colbase<true>(cb, ab + c1 + ccr, min(ccr, bd.nrows - c1 - ccr), wave);
+                load_offsets(vo, c1);
+            }
             store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, rb, K, cc1, wave, lane);
         }
         __syncthreads();
