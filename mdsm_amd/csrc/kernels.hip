@@ -391,11 +391,31 @@ __device__ __forceinline__ void fetch_chunk(PreRegs<DT> &pre, const void *src, i
         const int64_t rowbase = in_off + (row0 + i) * rs;
         const int col0 = cb[rr] + t0;
         const bool aligned = ((rowbase * es) & 3) == 0;
+        // Integer rows whose whole window is inside the row (every item but
+        // a row's last) take one address per row, entry column lane + 64*it
+        // at a constant offset from it, and no per-column bounds tests.  The
+        // test is wave-uniform, but is made per column on a value the
+        // compiler must treat as per-lane (the empty asm), so that both
+        // branches are exec-masked and load into the same registers: as a
+        // scalar branch the compiler merged the two paths by copying the
+        // loaded registers, waiting for the loads first.
+        constexpr bool kRowFast = DT == DT_U8 || DT == DT_U16;
+        int row_fast = kRowFast && aligned && col0 + 4 * K + 4 <= in_len;
+        if constexpr (kRowFast) asm volatile("" : "+v"(row_fast));
+        const MDSM_GLOBAL_AS uint32_t *q =
+            gptr<uint32_t>((const char *)src + (rowbase + col0) * es) + lane * es;
 #pragma unroll
         for (int it = 0; it < pf_iters<DT>(); ++it) {
             const int k = lane + 64 * it;
             if (64 * it >= K) break;
-            if (k < K) fetch8<DT>(src, rowbase, col0 + 4 * k, in_len, aligned, pre[rr][it]);
+            if (k < K) {
+                if (row_fast) {
+#pragma unroll
+                    for (int j = 0; j < Layout<DT>::raw; ++j) pre[rr][it][j] = q[64 * es * it + j];
+                } else {
+                    fetch8<DT>(src, rowbase, col0 + 4 * k, in_len, aligned, pre[rr][it]);
+                }
+            }
         }
     }
 }
