@@ -37,7 +37,21 @@ VALU_ADD_PEAK_T = 39.3         # unpacked fp32 adds/s (256 CU x 2.4 GHz x 64), T
 # kernel's read, 98.5 TB/s at 4 waves per SIMD and 109 TB/s at 8.
 LDS_PEAK_TBS = 137.6
 LDS_PEAK_SOURCE = "profiles/lds_probe.md: ds_read_b128, 8 waves per SIMD"
-LDS_B64_AT_4_WAVES_TBS = 98.5
+LDS_B64_AT_4_WAVES_TBS = 97.4
+# LDS write roof (mdsm_probe_lds_write): ds_write_b64 at 4 waves per SIMD
+LDS_WRITE_B64_AT_4_WAVES_TBS = 50.8
+
+
+def lds_roof(reads, writes, seconds):
+    """The dedispersion kernel's LDS roof at its occupancy (4 waves per SIMD):
+    the trial reads at the ds_read_b64 rate plus the staging stores at the
+    ds_write_b64 rate, against the measured time."""
+    roof_s = reads / (LDS_B64_AT_4_WAVES_TBS * 1e12) + writes / (LDS_WRITE_B64_AT_4_WAVES_TBS * 1e12)
+    return {"read_bytes": reads, "write_bytes": writes, "roof_ms": roof_s * 1e3,
+            "rw_frac": roof_s / seconds,
+            "rw_note": "LDS time of the trial reads (ds_read_b64, 97.4 TB/s) and staging stores "
+                       "(ds_write_b64, 50.8 TB/s) at 4 waves per SIMD, each at its probed rate "
+                       "(profiles/lds_probe.md), over the measured kernel time"}
 # BASELINE's headline metric: throughput, its fraction of the HBM roofline,
 # real-time factor and p99 block latency, against the plain float32 kernel
 BASELINE_METRIC = ("DM\u00b7chan\u00b7samples/s, %HBM roofline, RT factor, p99 block latency "
@@ -188,6 +202,7 @@ def serve(M, w, args):
     tr = read_traffic(prof)
     lds_bytes_blk = units_blk * 2.0
     occ = {"occupancy_theoretical": tp.occupancy(0)}
+    lds_rw = lds_roof(*tp.lds_bytes(0), ded_s_blk)
     # latency at the real arrival rate (host pushes: PCIe included)
     print("[bench] cfg5: latency at the real arrival rate", file=sys.stderr, flush=True)
 
@@ -291,7 +306,7 @@ def serve(M, w, args):
                 "achieved_tbs": lds_bytes_blk / ded_s_blk / 1e12, "peak_tbs": LDS_PEAK_TBS,
                 "frac": lds_bytes_blk / ded_s_blk / 1e12 / LDS_PEAK_TBS,
                 "peak_source": LDS_PEAK_SOURCE, "b64_at_4_waves_tbs": LDS_B64_AT_4_WAVES_TBS,
-                **occ},
+                **occ, **lds_rw},
         "kernels_ms_per_block": {k: v[0] / nblk for k, v in kern.items()},
         "mean_batch": mean_batch_thr, "pulse_blocks": pulses_thr, "tuning": tuning,
         "server_stats": {k: int(v) for k, v in st.items()},
@@ -845,7 +860,10 @@ def main():
         "lds": {"bytes_per_launch": lds_bytes,
                 "achieved_tbs": lds_bytes / avg_launch_s / 1e12, "peak_tbs": LDS_PEAK_TBS,
                 "frac": lds_bytes / avg_launch_s / 1e12 / LDS_PEAK_TBS,
-                "note": "binding resource: LDS gathers, one per DM*chan*sample"},
+                "note": "binding resource: LDS gathers, one per DM*chan*sample",
+                **lds_roof(*[sum(v) for v in zip(*[plan.lds_bytes(i) for i in
+                                                    range(2 if info.is_subband else 1)])],
+                           avg_launch_s)},
         "valu": {"adds_per_launch": adds_per_launch,
                  "achieved_tadds": adds_per_launch / avg_launch_s / 1e12,
                  "peak_tadds": VALU_ADD_PEAK_T,

@@ -199,6 +199,16 @@ typedef struct {
 
 int mdsm_plan_occupancy(mdsm_plan *plan, uint32_t stage, mdsm_occupancy *out);
 
+/* LDS bytes one launch of dedispersion stage `stage` moves: *reads, the
+ * trial reads (per trial slot, row and 256-sample tile: 2 bytes per sample
+ * for 16-bit staged rows, 4 for float32; the padding slots of a block's last
+ * trials and the whole last tile included),
+ * and *writes, the staging stores (every row's phase layout, once per tile).
+ * With the LDS read and write roofs (mdsm_probe_lds_read/_write) they give
+ * the time the LDS needs for the launch.  No counterpart in the reference. */
+int mdsm_plan_lds_bytes(const mdsm_plan *plan, uint32_t stage, uint64_t *reads,
+                        uint64_t *writes);
+
 int mdsm_plan_set_tuning(mdsm_plan *plan, const mdsm_tuning *t);
 int mdsm_plan_get_tuning(const mdsm_plan *plan, mdsm_tuning *t);
 
@@ -735,6 +745,11 @@ int mdsm_timing_get(const char *kernel, double *total_ms, uint64_t *launches);
  * The roof the dedispersion kernel's LDS rate is measured against.  No
  * counterpart in the reference. */
 int mdsm_probe_lds_read(uint32_t bytes_per_lane, uint32_t waves_per_simd, double *tbs);
+
+/* LDS write bandwidth (TB/s), the same way: ds_write_b32/_b64/_b128 of 64
+ * contiguous lanes at a wave-uniform moving offset (the dedispersion
+ * kernel's staging stores).  No counterpart in the reference. */
+int mdsm_probe_lds_write(uint32_t bytes_per_lane, uint32_t waves_per_simd, double *tbs);
 
 const char *mdsm_last_error(void);
 const char *mdsm_version(void);

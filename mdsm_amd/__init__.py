@@ -194,8 +194,10 @@ def lib():
                                 vp, u32, C.POINTER(Tuning)],
         "mdsm_plan_set_tuning": [vp, C.POINTER(Tuning)],
         "mdsm_plan_occupancy": [vp, u32, C.POINTER(Occupancy)],
+        "mdsm_plan_lds_bytes": [vp, u32, C.POINTER(C.c_uint64), C.POINTER(C.c_uint64)],
         "mdsm_plan_upload": [vp],
         "mdsm_probe_lds_read": [u32, u32, C.POINTER(f64)],
+        "mdsm_probe_lds_write": [u32, u32, C.POINTER(f64)],
         "mdsm_dedisperse_noalloc": [vp, vp, i64, vp, vp],
         "mdsm_plan_get_tuning": [vp, C.POINTER(Tuning)],
         "mdsm_plan_autotune": [vp, vp, i64, vp, vp, vp, u32, C.POINTER(Tuning)],
@@ -492,6 +494,13 @@ class Plan:
         check(lib().mdsm_plan_occupancy(self._h, int(stage), C.byref(o)))
         return {f: getattr(o, f) for f, _ in Occupancy._fields_}
 
+    def lds_bytes(self, stage: int = 0) -> tuple:
+        """(read, write) LDS bytes of one launch of dedispersion `stage`: the
+        trial reads and the staging stores."""
+        r, w = C.c_uint64(), C.c_uint64()
+        check(lib().mdsm_plan_lds_bytes(self._h, int(stage), C.byref(r), C.byref(w)))
+        return r.value, w.value
+
     def split_output(self, flat: np.ndarray) -> list:
         return [flat[R.out_offset:R.out_offset + R.ndms * R.nout].reshape(R.ndms, R.nout)
                 for R in self.ranges]
@@ -692,6 +701,13 @@ def probe_lds_read(bytes_per_lane: int, waves_per_simd: int) -> float:
     """Measured LDS read bandwidth (TB/s) of the current GPU."""
     v = C.c_double()
     check(lib().mdsm_probe_lds_read(int(bytes_per_lane), int(waves_per_simd), C.byref(v)))
+    return v.value
+
+
+def probe_lds_write(bytes_per_lane: int, waves_per_simd: int) -> float:
+    """Measured LDS write bandwidth (TB/s) of the current GPU."""
+    v = C.c_double()
+    check(lib().mdsm_probe_lds_write(int(bytes_per_lane), int(waves_per_simd), C.byref(v)))
     return v.value
 
 

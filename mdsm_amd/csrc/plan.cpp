@@ -846,6 +846,30 @@ extern "C" int mdsm_plan_occupancy(mdsm_plan *p, uint32_t stage, mdsm_occupancy 
     return MDSM_OK;
 }
 
+extern "C" int mdsm_plan_lds_bytes(const mdsm_plan *p, uint32_t stage, uint64_t *reads,
+                                   uint64_t *writes)
+{
+    if (!p || !reads || !writes) return fail(MDSM_ERR_INVALID, "NULL argument");
+    if (stage >= p->stages.size()) return fail(MDSM_ERR_INVALID, "no such stage");
+    const Stage &st = p->stages[stage];
+    uint64_t r = 0, w = 0;
+    for (int si = st.seg_begin; si < st.seg_end; ++si) {
+        const SegDesc &sg = p->segs[si];
+        for (int bi = sg.first_block; bi < sg.first_block + sg.nblocks; ++bi) {
+            const BlockDesc &b = p->blocks[bi];
+            const uint64_t rows = (uint64_t)b.nrows * (uint64_t)sg.ntiles;
+            // every (row, trial slot) of a tile: 8 bytes of LDS per lane for
+            // 16-bit entries, 16 for float32 (padding slots read too)
+            const uint64_t per_sample = b.dtype == DT_F32 ? 4 : 2;
+            r += rows * (uint64_t)(kWaves * b.rpw) * 256u * per_sample;
+            w += rows * (uint64_t)lds_row_bytes(b.dtype, b.W);
+        }
+    }
+    *reads = r;
+    *writes = w;
+    return MDSM_OK;
+}
+
 extern "C" int mdsm_plan_set_tuning(mdsm_plan *p, const mdsm_tuning *t)
 {
     if (!p || !t) return fail(MDSM_ERR_INVALID, "NULL argument");

@@ -1,5 +1,5 @@
-// LDS read-bandwidth probe: the roof the dedispersion kernel is measured
-// against.  Each wave reads 64 lanes x `W` contiguous bytes per instruction
+// LDS read- and write-bandwidth probes: the roofs the dedispersion kernel's
+// trial reads and staging stores are measured against.  Each wave reads 64 lanes x `W` contiguous bytes per instruction
 // (ds_read_b32 / _b64 / _b128) at a wave-uniform offset that moves on every
 // read, the access shape of the dedispersion kernel's trial reads; the offset
 // is masked with a run-time mask so that the compiler cannot merge reads.
@@ -46,9 +46,41 @@ __global__ void __launch_bounds__(256) lds_probe_kernel(int iters, uint32_t mask
     out[blockIdx.x * 256 + threadIdx.x] = acc;
 }
 
-}  // namespace
+// Write form: each wave writes 64 lanes x W contiguous bytes per
+// instruction (ds_write_b32 / _b64 / _b128) at a wave-uniform offset that
+// moves on every write, the access shape of the dedispersion kernel's
+// staging stores.  Every address is a multiple of 64*W bytes (of W for the
+// lane part) inside the masked region; the barrier and the read-back keep
+// the stores live.
+template <int W>
+__global__ void __launch_bounds__(256) lds_write_probe_kernel(int iters, uint32_t mask, uint32_t *out)
+{
+    extern __shared__ __attribute__((aligned(16))) unsigned char lds[];
+    const uint32_t lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    uint32_t v = threadIdx.x * 2654435761u;
+    uint32_t off = wave * 1024u;
+    constexpr uint32_t kAlign = 64u * W;
+    for (int it = 0; it < iters; ++it) {
+#pragma unroll
+        for (int u = 0; u < kProbeUnroll; ++u) {
+            const uint32_t a = ((off + (uint32_t)u * 528u) & mask & ~(kAlign - 1)) + lane * W;
+            if constexpr (W == 4)
+                *(uint32_t *)(lds + a) = v + u;
+            else if constexpr (W == 8)
+                *(uint2 *)(lds + a) = make_uint2(v + u, v ^ u);
+            else
+                *(uint4 *)(lds + a) = make_uint4(v + u, v ^ u, v, u);
+        }
+        off += kProbeUnroll * 528u + 16u;
+        v += 1u;
+    }
+    __syncthreads();
+    out[blockIdx.x * 256 + threadIdx.x] = ((const uint32_t *)lds)[threadIdx.x];
+}
 
-extern "C" int mdsm_probe_lds_read(uint32_t bytes_per_lane, uint32_t waves_per_simd, double *tbs)
+// Shared driver of the two probes: occupancy set through the workgroup's LDS
+// allocation, one warm-up launch, one timed launch.
+int probe_lds(bool write, uint32_t bytes_per_lane, uint32_t waves_per_simd, double *tbs)
 {
     if (!tbs) return fail(MDSM_ERR_INVALID, "NULL argument");
     if (bytes_per_lane != 4 && bytes_per_lane != 8 && bytes_per_lane != 16)
@@ -71,9 +103,14 @@ extern "C" int mdsm_probe_lds_read(uint32_t bytes_per_lane, uint32_t waves_per_s
     uint32_t *out = nullptr;
     e = hipMalloc(&out, (size_t)grid * 256 * 4);
     if (e != hipSuccess) return hip_fail(e, "probe buffer");
-    void (*k)(int, uint32_t, uint32_t *) =
-        bytes_per_lane == 4 ? lds_probe_kernel<4>
-                            : (bytes_per_lane == 8 ? lds_probe_kernel<8> : lds_probe_kernel<16>);
+    void (*k)(int, uint32_t, uint32_t *) = nullptr;
+    if (write)
+        k = bytes_per_lane == 4 ? lds_write_probe_kernel<4>
+                                : (bytes_per_lane == 8 ? lds_write_probe_kernel<8>
+                                                       : lds_write_probe_kernel<16>);
+    else
+        k = bytes_per_lane == 4 ? lds_probe_kernel<4>
+                                : (bytes_per_lane == 8 ? lds_probe_kernel<8> : lds_probe_kernel<16>);
     e = hipFuncSetAttribute((const void *)k, hipFuncAttributeMaxDynamicSharedMemorySize, 163840);
     hipEvent_t e0 = nullptr, e1 = nullptr;
     if (e == hipSuccess) e = hipEventCreate(&e0);
@@ -95,4 +132,16 @@ extern "C" int mdsm_probe_lds_read(uint32_t bytes_per_lane, uint32_t waves_per_s
     const double bytes = (double)grid * 256.0 * iters * kProbeUnroll * bytes_per_lane;
     *tbs = bytes / (ms * 1e-3) / 1e12;
     return MDSM_OK;
+}
+
+}  // namespace
+
+extern "C" int mdsm_probe_lds_read(uint32_t bytes_per_lane, uint32_t waves_per_simd, double *tbs)
+{
+    return probe_lds(false, bytes_per_lane, waves_per_simd, tbs);
+}
+
+extern "C" int mdsm_probe_lds_write(uint32_t bytes_per_lane, uint32_t waves_per_simd, double *tbs)
+{
+    return probe_lds(true, bytes_per_lane, waves_per_simd, tbs);
 }

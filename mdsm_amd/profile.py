@@ -63,14 +63,16 @@ def _serve(steps: int, warmup: int) -> None:
 
 
 def _lds_probe() -> None:
-    """LDS read bandwidth of ds_read_b32/_b64/_b128 at 1, 2, 4 and 8 waves per SIMD."""
+    """LDS read and write bandwidth of ds_read/ds_write _b32/_b64/_b128 at 1,
+    2, 4 and 8 waves per SIMD."""
     import mdsm_amd as M
-    print("| bytes per lane | waves per SIMD | TB/s |")
-    print("|---|---|---|")
+    print("| bytes per lane | waves per SIMD | read TB/s | write TB/s |")
+    print("|---|---|---|---|")
     for w in (4, 8, 16):
         for occ in (1, 2, 4, 8):
-            v = max(M.probe_lds_read(w, occ) for _ in range(3))
-            print(f"| {w} | {occ} | {v:.1f} |", flush=True)
+            r = max(M.probe_lds_read(w, occ) for _ in range(3))
+            wr = max(M.probe_lds_write(w, occ) for _ in range(3))
+            print(f"| {w} | {occ} | {r:.1f} | {wr:.1f} |", flush=True)
 
 
 def _copy() -> None:
