@@ -33,9 +33,9 @@ import numpy as np
 HBM_PEAK_GBS = 8000.0          # MI355X HBM3E, GB/s
 VALU_ADD_PEAK_T = 39.3         # unpacked fp32 adds/s (256 CU x 2.4 GHz x 64), T/s
 # LDS read roof, measured on MI355X (profiles/lds_probe.md, mdsm_probe_lds_read):
-# ds_read_b128 at 8 waves per SIMD 137.6 TB/s; ds_read_b64, the integer
-# kernel's read, 98.5 TB/s at 4 waves per SIMD and 109 TB/s at 8.
-LDS_PEAK_TBS = 137.6
+# ds_read_b128 at 8 waves per SIMD 135.1 TB/s; ds_read_b64, the integer
+# kernel's read, 97.4 TB/s at 4 waves per SIMD and 108 TB/s at 8.
+LDS_PEAK_TBS = 135.1
 LDS_PEAK_SOURCE = "profiles/lds_probe.md: ds_read_b128, 8 waves per SIMD"
 LDS_B64_AT_4_WAVES_TBS = 97.4
 # LDS write roof (mdsm_probe_lds_write): ds_write_b64 at 4 waves per SIMD
