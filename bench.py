@@ -861,9 +861,11 @@ def main():
                 "achieved_tbs": lds_bytes / avg_launch_s / 1e12, "peak_tbs": LDS_PEAK_TBS,
                 "frac": lds_bytes / avg_launch_s / 1e12 / LDS_PEAK_TBS,
                 "note": "binding resource: LDS gathers, one per DM*chan*sample",
-                **lds_roof(*[sum(v) for v in zip(*[plan.lds_bytes(i) for i in
-                                                    range(2 if info.is_subband else 1)])],
-                           avg_launch_s)},
+                # the read+write roof for integer rows (ds_read_b64 trial reads);
+                # float32 rows read through ds_read2st64_b64 at another rate
+                **(lds_roof(*[sum(v) for v in zip(*[plan.lds_bytes(i) for i in
+                                                     range(2 if info.is_subband else 1)])],
+                            avg_launch_s) if w["dtype"] == "u8" else {})},
         "valu": {"adds_per_launch": adds_per_launch,
                  "achieved_tadds": adds_per_launch / avg_launch_s / 1e12,
                  "peak_tadds": VALU_ADD_PEAK_T,
