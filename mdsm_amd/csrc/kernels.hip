@@ -1531,6 +1531,14 @@ constexpr int kBR = kBW - 4 - 64;       // positions reported per wave (maxw <= 
 
 __device__ __forceinline__ double shfl_down_d(double v, int k) { return __shfl_down(v, k, 64); }
 __device__ __forceinline__ double shfl_up_d(double v, int k) { return __shfl_up(v, k, 64); }
+__device__ __forceinline__ float shfl_down_d(float v, int k) { return __shfl_down(v, k, 64); }
+__device__ __forceinline__ float shfl_up_d(float v, int k) { return __shfl_up(v, k, 64); }
+// v of lane i+1 (lane 63 keeps its own): DPP wave_shl:1
+__device__ __forceinline__ float next_lane_d(float v)
+{
+    const int u = __builtin_bit_cast(int, v);
+    return __builtin_bit_cast(float, __builtin_amdgcn_update_dpp(u, u, 0x130, 0xf, 0xf, false));
+}
 // v of lane i+1 (lane 63 keeps its own): DPP wave_shl:1 on both halves
 __device__ __forceinline__ double next_lane_d(double v)
 {
@@ -1541,11 +1549,21 @@ __device__ __forceinline__ double next_lane_d(double v)
     return __builtin_bit_cast(double, ((uint64_t)nhi << 32) | nlo);
 }
 
+// T = float: for series of integers whose window sums stay below 2^24 (the
+// caller guarantees it: uint8 data summed over C channels, 255 C maxw < 2^24),
+// every sum of the tree is exact in float32 and equal to the FP64 one.  The
+// pre-filter then runs in float32 with a margin that covers its two
+// roundings (below 1 + 2^-23 |lo| at these magnitudes; 4 + 2^-20 |lo| is
+// used), so every sample the FP64 pre-filter passes also passes it, and the
+// passing samples are decided in FP64 exactly as in the double form: the
+// candidates are the same.
+template <typename T>
 __global__ void __launch_bounds__(256)
     boxcar_reg_kernel(const float *x, uint32_t n, int64_t stride, BoxStream bs, uint32_t wmask,
                       uint32_t maxw, double thr, SqrtW sqrtw, mdsm_cand *cands, uint32_t cap,
                       uint32_t *count, const double *mean, const double *sigma)
 {
+    constexpr bool kF32 = sizeof(T) == 4;
     const uint32_t d = blockIdx.y;
     const double sg = sigma[d];
     if (!(sg > 0.0)) return;
@@ -1559,7 +1577,7 @@ __global__ void __launch_bounds__(256)
     const float *crow = bs.carry + (int64_t)d * bs.carry_stride;
     const int64_t b = T0 - 4;       // sample held by lane 0, slot 0
     const int64_t tl = b + kBL * lane;
-    double s[kBL];
+    T s[kBL];
     // 16-byte loads when every float4 of the lane's 16 samples is aligned and
     // lies in one source (H, tl and both row bases are multiples of 4 floats)
     const bool vec = tl >= 0 && tl + kBL <= ne && ((tl | H) & 3) == 0 &&
@@ -1579,7 +1597,7 @@ __global__ void __launch_bounds__(256)
 #pragma unroll
         for (int j = 0; j < kBL; ++j) {
             const int64_t t = tl + j;
-            s[j] = (t >= 0 && t < ne) ? (double)(t < H ? crow[t] : row[t - H]) : 0.0;
+            s[j] = (t >= 0 && t < ne) ? (T)(t < H ? crow[t] : row[t - H]) : (T)0;
         }
     }
 #pragma unroll
@@ -1591,6 +1609,8 @@ __global__ void __launch_bounds__(256)
             const double den = sg * sqrtw.v[lv];
             const double td = thr * den;
             const double lo = td - fabs(td) * 0x1p-40;  // exact pre-filter, as above
+            const float wm32 = (float)wm;
+            const float lo32 = (float)lo - (4.0f + fabsf((float)lo) * 0x1p-20f);
             const int64_t last = ne - w;
             const int64_t tmax = last - bs.tail;
             const int64_t tmin = H > (int64_t)w ? H - w : 0;
@@ -1600,21 +1620,26 @@ __global__ void __launch_bounds__(256)
             for (int j = 0; j < kBL; ++j) {
                 const int u = kBL * lane + j;
                 const int64_t t = b + u;
-                c[j] = u >= 4 && u < 4 + kBR && t <= tmax && t >= tmin && (s[j] - wm) > lo;
+                bool pass;
+                if constexpr (kF32)
+                    pass = (s[j] - wm32) > lo32;
+                else
+                    pass = (s[j] - wm) > lo;
+                c[j] = u >= 4 && u < 4 + kBR && t <= tmax && t >= tmin && pass;
                 any = any || c[j];
             }
             if (__any(any)) {
-                const double prevL = shfl_up_d(s[kBL - 1], 1);
-                const double nextF = next_lane_d(s[0]);
+                const double prevL = (double)shfl_up_d(s[kBL - 1], 1);
+                const double nextF = (double)next_lane_d(s[0]);
 #pragma unroll
                 for (int j = 0; j < kBL; ++j) {
                     if (!c[j]) continue;
                     const int64_t t = b + kBL * lane + j;
-                    const double sv = s[j];
+                    const double sv = (double)s[j];
                     const double snr = (sv - wm) / den;
                     if (!(snr > thr)) continue;
-                    const double pv = j > 0 ? s[j > 0 ? j - 1 : 0] : prevL;
-                    const double nv = j < kBL - 1 ? s[j < kBL - 1 ? j + 1 : 0] : nextF;
+                    const double pv = j > 0 ? (double)s[j > 0 ? j - 1 : 0] : prevL;
+                    const double nv = j < kBL - 1 ? (double)s[j < kBL - 1 ? j + 1 : 0] : nextF;
                     if (t > 0 && !(sv >= pv)) continue;
                     if (t < last && !(sv > nv)) continue;
                     const uint32_t slot = atomicAdd(count, 1u);
@@ -1632,7 +1657,7 @@ __global__ void __launch_bounds__(256)
         if (2 * w > maxw) break;
         // next level, S_2w(t) = S_w(t) + S_w(t + w), reading only old values
         if (w < (uint32_t)kBL) {
-            double nx[kBL];
+            T nx[kBL];
 #pragma unroll
             for (int j = 0; j < kBL; ++j)
                 if (j < (int)w) nx[j] = next_lane_d(s[j]);
@@ -1666,7 +1691,7 @@ hipError_t launch_boxcar_stream(const float *x, uint32_t ndms, uint32_t n, int64
                                 const BoxStream &bs, uint32_t wmask, double thr,
                                 const SqrtW &sqrtw, mdsm_cand *cands, uint32_t cap,
                                 uint32_t *count, double *mean, double *sigma, bool stats,
-                                hipStream_t s)
+                                hipStream_t s, bool int_exact)
 {
     hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, count, 1u, 0u);
     if (ndms == 0 || n == 0) return hipGetLastError();
@@ -1680,8 +1705,14 @@ hipError_t launch_boxcar_stream(const float *x, uint32_t ndms, uint32_t n, int64
     const int64_t ne = (int64_t)n + bs.H;
     if (maxw <= 64) {
         const uint32_t tiles = (uint32_t)((ne + kBR - 1) / kBR);
-        hipLaunchKernelGGL(boxcar_reg_kernel, dim3((tiles + 3) / 4, ndms), dim3(256), 0, s, x, n,
-                           stride, bs, wmask, maxw, thr, sqrtw, cands, cap, count, mean, sigma);
+        if (int_exact)
+            hipLaunchKernelGGL(boxcar_reg_kernel<float>, dim3((tiles + 3) / 4, ndms), dim3(256), 0,
+                               s, x, n, stride, bs, wmask, maxw, thr, sqrtw, cands, cap, count,
+                               mean, sigma);
+        else
+            hipLaunchKernelGGL(boxcar_reg_kernel<double>, dim3((tiles + 3) / 4, ndms), dim3(256),
+                               0, s, x, n, stride, bs, wmask, maxw, thr, sqrtw, cands, cap, count,
+                               mean, sigma);
         return hipGetLastError();
     }
     dim3 grid((uint32_t)((ne + kBT - 1) / kBT), ndms);

@@ -280,11 +280,14 @@ uint32_t boxcar_maxw(uint32_t wmask);
 // up to a multiple of 4 (so that 16-byte loads never straddle the two sources).
 uint32_t boxcar_carry_len(uint32_t wmask);
 // stats = false: mean/sigma already hold the row statistics of the block.
+// int_exact: the caller guarantees the series (carry included) are integers
+// whose sums over maxw samples stay below 2^24 (then the tree runs exactly in
+// float32; same candidates).
 hipError_t launch_boxcar_stream(const float *x, uint32_t ndms, uint32_t n, int64_t stride,
                                 const BoxStream &bs, uint32_t wmask, double thr,
                                 const SqrtW &sqrtw, mdsm_cand *cands, uint32_t cap,
                                 uint32_t *count, double *mean, double *sigma, bool stats,
-                                hipStream_t s);
+                                hipStream_t s, bool int_exact = false);
 hipError_t launch_boxcar(const float *x, uint32_t ndms, uint32_t n, int64_t stride,
                          uint32_t wmask, double thr, const SqrtW &sqrtw, mdsm_cand *cands,
                          uint32_t cap, uint32_t *count, double *mean, double *sigma,

@@ -455,13 +455,19 @@ int launch_batch(mdsm_server *s, Batch *bt)
                 const uint32_t n = b->N / ri.downsample;
                 float *carry = b->carry + (size_t)ri.first_dm * H;
                 const BoxStream bs{carry, (int64_t)H, j->carry_valid ? (int32_t)H : 0, 1};
+                // uint8 ranges at full rate: the series are integer sums of at
+                // most 255 per channel (the clipper writes integer values), so
+                // window sums below 2^24 are exact in float32
+                const bool int_exact =
+                    b->obs.dtype == MDSM_U8 && ri.downsample == 1 &&
+                    255.0 * (double)C * (double)boxcar_maxw(b->wmask) < 16777216.0;
                 {
                     TimedLaunch tl("detect", st);
                     e = launch_boxcar_stream(j->out + ri.out_offset, ri.ndms, n, ri.nout, bs,
                                              b->wmask, b->det.snr_thr, sw,
                                              j->cands + (size_t)r * cap, cap, j->count + r,
                                              j->mean + ri.first_dm, j->sigma + ri.first_dm, true,
-                                             st);
+                                             st, int_exact);
                 }
                 if (e != hipSuccess) return hip_fail(e, "server detect");
                 // the last H samples of each row carry over to the next block

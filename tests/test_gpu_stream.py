@@ -262,3 +262,45 @@ def test_device_push_matches_host_push(gpu):
         srv.close()
     for (o1, c1, _), (o2, c2, _) in zip(*outs):
         assert np.array_equal(o1, o2) and _key(c1) == _key(c2)
+
+
+def test_bright_wide_beam_detector_exact_at_the_float32_bound(gpu):
+    """1024 uint8 channels near full scale (mean 240): the window sums of the
+    widest boxcar reach 1.6e7, just under 2^24, where the server runs its
+    detector's sum tree in float32 (exact for these integer series).  Every
+    block's output, and its candidates with their S/N, equal the oracle's
+    FP64 streaming search."""
+    from mdsm_amd.server import Server
+    M = gpu
+    print("case server bright 1024-channel beam", flush=True)
+    C2, N2, nblk = 1024, 2048, 3
+    ranges = ((0.0, 1.0, 32, 1),)
+    assert 255 * C2 * 64 < 1 << 24
+    x = M.synth_filterbank(C2, N2 * nblk, "u8", seed=4242, mean=240, sigma=8,
+                           fch1=1500.0, foff=-300.0 / C2,
+                           pulses=[(12.0, 900, 6, 1.0), (20.0, N2 + 300, 40, 0.6),
+                                   (8.0, 2 * N2 - 200, 64, 0.5)])
+    srv = Server(max_batch=2)
+    b = srv.open_beam(**_beam_kwargs(C2, N2, ranges))
+    res = {}
+    for k in range(nblk):
+        blk = np.ascontiguousarray(x[:, k * N2:(k + 1) * N2])
+        while not srv.push_seq(b, blk, k):
+            srv.drain()
+            _collect(srv, res)
+    srv.drain()
+    _collect(srv, res)
+    ref = _oracle_beam(x, b, N2, nblk, ranges, C2)
+    ncand = 0
+    for k in range(nblk):
+        out, c, r = res[(b.id, k)]
+        assert r.status == 0
+        assert np.array_equal(out, ref[k][0]), f"block {k}: output != oracle"
+        got, exp = _key(c), _key(ref[k][1])
+        assert [t[:3] for t in got] == [t[:3] for t in exp], f"block {k}: candidates != oracle"
+        for g, e in zip(got, exp):
+            assert abs(g[3] - e[3]) <= 1e-6 * abs(e[3]), f"block {k}: S/N {g} != {e}"
+        ncand += len(got)
+    assert ncand > 0
+    assert float(np.max(ref[0][0])) * 64 > 0.9 * (1 << 24), "sums do not approach the bound"
+    srv.close()
