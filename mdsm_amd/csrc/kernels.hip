@@ -359,29 +359,41 @@ constexpr int pf_iters() { return max_window(DT) / 256; }
 template <int DT>
 using PreRegs = uint32_t[kPfRows][pf_iters<DT>()][Layout<DT>::raw];
 
-// Window start (sample column) of each row a wave stages in a chunk: a
-// vector load (a scalar one would tie the prefetch to the LDS waits).  The
-// uint8 path loads it a chunk ahead of its use, through a global-address-space
-// view; the other paths load it just before the prefetch through a generic
-// pointer, whose flat load waits for both counters, the scalar offsets of the
-// chunk's first row (pair) included (measured faster for them).
+// The staged window of one row: its first sample column, and its info --
+// bits 0-9 the window's entries per phase (a quarter of its samples), bits
+// 10- the row's byte place in its chunk buffer.
+struct RowWin {
+    int base, info;
+};
+
+// Windows of the rows a wave stages in a chunk: a vector load (a scalar one
+// would tie the prefetch to the LDS waits).  The uint8 path loads them a
+// chunk ahead of their use, through a global-address-space view; the other
+// paths load them just before the prefetch through a generic pointer, whose
+// flat load waits for both counters, the scalar offsets of the chunk's first
+// row (pair) included (measured faster for them).
 template <bool kAhead>
-__device__ __forceinline__ void load_colbase(int (&cb)[kPfRows], const int32_t *ab, int cc, int wave)
+__device__ __forceinline__ void load_colbase(RowWin (&cb)[kPfRows], const int32_t *ab, int cc, int wave)
 {
 #pragma unroll
     for (int rr = 0; rr < kPfRows; ++rr) {
         const int i = wave + kWaves * rr;
-        if constexpr (kAhead)
-            cb[rr] = i < cc ? gptr<int32_t>(ab)[i] : 0;
-        else
-            cb[rr] = i < cc ? ab[i] : 0;
+        int2 v = make_int2(0, 0);
+        if (i < cc) {
+            if constexpr (kAhead)
+                v = gptr<int2>(ab)[i];
+            else
+                v = ((const int2 *)ab)[i];
+        }
+        cb[rr].base = v.x;
+        cb[rr].info = v.y;
     }
 }
 
 template <int DT>
 __device__ __forceinline__ void fetch_chunk(PreRegs<DT> &pre, const void *src, int64_t in_off,
-                                            int64_t row0, int64_t rs, const int (&cb)[kPfRows],
-                                            int t0, int K, int cc, int in_len, int wave, int lane)
+                                            int64_t row0, int64_t rs, const RowWin (&cb)[kPfRows],
+                                            int t0, int cc, int in_len, int wave, int lane)
 {
     constexpr int es = Layout<DT>::es;
 #pragma unroll
@@ -389,7 +401,8 @@ __device__ __forceinline__ void fetch_chunk(PreRegs<DT> &pre, const void *src, i
         const int i = wave + kWaves * rr;
         if (i >= cc) break;
         const int64_t rowbase = in_off + (row0 + i) * rs;
-        const int col0 = cb[rr] + t0;
+        const int col0 = cb[rr].base + t0;
+        const int K = cb[rr].info & 1023;
         const bool aligned = ((rowbase * es) & 3) == 0;
         // Integer rows whose whole window is inside the row (every item but
         // a row's last) take one address per row, entry column lane + 64*it
@@ -421,14 +434,15 @@ __device__ __forceinline__ void fetch_chunk(PreRegs<DT> &pre, const void *src, i
 }
 
 template <int DT>
-__device__ __forceinline__ void store_chunk(const PreRegs<DT> &pre, char *buf, int rb, int K,
-                                            int cc, int wave, int lane)
+__device__ __forceinline__ void store_chunk(const PreRegs<DT> &pre, char *buf,
+                                            const RowWin (&cb)[kPfRows], int cc, int wave, int lane)
 {
 #pragma unroll
     for (int rr = 0; rr < kPfRows; ++rr) {
         const int i = wave + kWaves * rr;
         if (i >= cc) break;
-        char *row = buf + (size_t)i * rb;
+        char *row = buf + (cb[rr].info >> 10);
+        const int K = cb[rr].info & 1023;
         if constexpr (DT == DT_F32) {
 #pragma unroll
             for (int it = 0; it < pf_iters<DT>(); ++it) {
@@ -513,14 +527,12 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
 {
     constexpr bool kInt = DT == DT_U8 || DT == DT_U16;
     constexpr int kEB = Layout<DT>::eb;
-    const int W = bd.W, K = W >> 2;
     const int64_t rs = bd.src == BUF_IN ? a.in_stride : bd.row_stride;
     const void *src = bd.src == BUF_IN ? a.in : (const void *)a.work;
     const int32_t *relb = uniform_ptr(a.rel + bd.tab_off);  // [nrows][kDB] (+8 zero rows)
-    const int32_t *ab = uniform_ptr(a.abase + bd.base_off);
+    const int32_t *ab = uniform_ptr(a.abase + bd.base_off);  // [nrows] (start, info) pairs
     const int ccr = bd.ccr;
-    const int rb = 4 * K * kEB;                 // bytes per staged row
-    const size_t bufsz = (size_t)ccr * rb;      // bytes per chunk buffer
+    const size_t bufsz = (size_t)bd.bufsz;      // bytes per chunk buffer
     const int nch = (bd.nrows + ccr - 1) / ccr;
 
     float f[R][4];
@@ -749,15 +761,15 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         load_offsets(vo, 0);
     else
         load_soffsets(relb + wave * kRPW);
-    int cb[kPfRows];  // window starts of the rows the next prefetch stages
+    RowWin cb[kPfRows];  // windows of the rows the next prefetch stages
     __syncthreads();  // the previous item (persistent mode) is done with the buffers
     {
         const int cc0 = min(ccr, bd.nrows);
         load_colbase<kVOff>(cb, ab, cc0, wave);
-        fetch_chunk<DT>(pre, src, bd.in_off, bd.row0, rs, cb, t0, K, cc0, bd.in_len, wave, lane);
-        store_chunk<DT>(pre, lds, rb, K, cc0, wave, lane);
+        fetch_chunk<DT>(pre, src, bd.in_off, bd.row0, rs, cb, t0, cc0, bd.in_len, wave, lane);
+        store_chunk<DT>(pre, lds, cb, cc0, wave, lane);
         if constexpr (kVOff)
-            if (nch > 1) load_colbase<true>(cb, ab + ccr, min(ccr, bd.nrows - ccr), wave);
+            if (nch > 1) load_colbase<true>(cb, ab + 2 * ccr, min(ccr, bd.nrows - ccr), wave);
     }
     __syncthreads();
     for (int ch = 0; ch < nch; ++ch) {
@@ -772,9 +784,18 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         // registers from waiting for the prefetch below (the counter is in
         // order).
         if constexpr (kVOff) __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
+        // the windows of the rows the prefetch below stages, kept (wave-
+        // uniform, in scalar registers) for their staging stores: the uint8
+        // path reloads cb for the chunk after before those stores
+        RowWin cs[kPfRows];
         if (more) {  // next chunk's global loads overlap this chunk's compute
-            if constexpr (!kVOff) load_colbase<false>(cb, ab + c1, cc1, wave);
-            fetch_chunk<DT>(pre, src, bd.in_off, bd.row0 + c1, rs, cb, t0, K, cc1, bd.in_len,
+            if constexpr (!kVOff) load_colbase<false>(cb, ab + 2 * c1, cc1, wave);
+#pragma unroll
+            for (int rr = 0; rr < kPfRows; ++rr) {
+                cs[rr].base = __builtin_amdgcn_readfirstlane(cb[rr].base);
+                cs[rr].info = __builtin_amdgcn_readfirstlane(cb[rr].info);
+            }
+            fetch_chunk<DT>(pre, src, bd.in_off, bd.row0 + c1, rs, cs, t0, cc1, bd.in_len,
                             wave, lane);
         }
         if constexpr (kInt) {
@@ -793,10 +814,11 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
             // in flight across the staging stores and the barrier
             if constexpr (kVOff) {
                 if (ch + 2 < nch)
-                    load_colbase<true>(cb, ab + c1 + ccr, min(ccr, bd.nrows - c1 - ccr), wave);
+                    load_colbase<true>(cb, ab + 2 * (c1 + ccr), min(ccr, bd.nrows - c1 - ccr),
+                                       wave);
                 load_offsets(vo, c1);
             }
-            store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, rb, K, cc1, wave, lane);
+            store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, cs, cc1, wave, lane);
         }
         __syncthreads();
     }

@@ -56,16 +56,19 @@ struct BlockDesc {
     int32_t in_len;           // valid columns per row
     int32_t nrows;
     int32_t ntrials;
-    int32_t W;                // LDS row width (floats), multiple of 64
+    int32_t W;                // widest row window of the block (samples), multiple of 64
     int32_t nout;
     int32_t k;                // divisor applied at store (1 = none)
     int32_t src, dst;
     int32_t dtype;
     int32_t tab_off;          // int offset of rel[nrows][kDB] in the rel table
-    int32_t base_off;         // int offset of abase[nrows] in the base table
+    int32_t base_off;         // int offset of the block's (window start, row info) pairs in
+                              // the base table: info bits 0-9 entries per phase of the row's
+                              // window, bits 10- its byte place in the chunk buffer
     int32_t ccr;              // rows staged per LDS chunk
     int32_t flush;            // integer path: rows summed in 16-bit lanes between flushes
     int32_t rpw;              // trials per wave: kRPW, or kRPW/2 for float blocks of <= kDB/2 trials
+    int32_t bufsz;            // bytes of one chunk buffer (its rows at their own widths)
 };
 
 // LDS bytes of one staged row: integer data (uint8/uint16) is held as four

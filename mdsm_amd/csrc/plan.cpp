@@ -100,7 +100,7 @@ int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
     SegDesc seg{};
     seg.first_block = (int32_t)p->blocks.size();
     seg.ntiles = (int32_t)((s.nout + kTile - 1) / kTile);
-    std::vector<int32_t> base(s.nrows);
+    std::vector<int32_t> base(s.nrows), rspread(s.nrows);
     int j0 = 0;
     while (j0 < s.ntrials) {
         int nt = std::min(kDB, s.ntrials - j0);
@@ -116,6 +116,7 @@ int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
                 }
                 if (lo < 0) return fail(MDSM_ERR_INVALID, "negative shift in plan");
                 base[i] = lo & ~15;
+                rspread[i] = hi - base[i];
                 relmax = std::max<int64_t>(relmax, hi - base[i]);
             }
             if (round_up(kTile + relmax, 64) <= max_window(s.dtype) || nt == 1) break;
@@ -151,12 +152,20 @@ int add_segment(mdsm_plan *p, Stage &st, const SegSpec &s)
         b.rpw = (nt <= kDB / 2 && !int_path(s.dtype)) ? kRPW / 2 : kRPW;
         std::vector<int> col_trial(kDB, -1);
         for (int t = 0; t < nt; ++t) col_trial[(t / b.rpw) * kRPW + t % b.rpw] = t;
+        // Each row is staged over its own window: the tile plus its own
+        // spread of shifts over the block's trials (rows of the upper band
+        // spread far less than the block's widest), rounded up to 64
+        // samples.  The base table holds (window start, row info) pairs; the
+        // row info carries the window's entries per phase (bits 0-9) and,
+        // from apply_tuning, the row's byte place in its chunk buffer.
         for (int i = 0; i < s.nrows; ++i) {
+            const int Wi = (int)round_up(kTile + rspread[i], 64);
             p->abase.push_back(base[i]);
+            p->abase.push_back(Wi / 4);
             for (int j = 0; j < kDB; ++j) {
                 const int t = col_trial[j];
                 const int32_t o = t >= 0 ? s.sh(j0 + t, i) - base[i] : 0;
-                p->rel0.push_back(lds_offset(s.dtype, b.W, o));
+                p->rel0.push_back(lds_offset(s.dtype, Wi, o));
             }
         }
         st.wmax = std::max(st.wmax, b.W);
@@ -198,27 +207,32 @@ int add_decimation(mdsm_plan *p, int k, uint64_t &work)
 void apply_tuning(mdsm_plan *p)
 {
     const int budget = p->tune.lds_budget ? (int)p->tune.lds_budget : kLdsBudget;
+    p->rel = p->rel0;
     for (Stage &st : p->stages) {
         st.lds_bytes = 0;
         for (int si = st.seg_begin; si < st.seg_end; ++si) {
             const SegDesc &sg = p->segs[si];
             for (int bi = sg.first_block; bi < sg.first_block + sg.nblocks; ++bi) {
                 BlockDesc &b = p->blocks[bi];
-                const int rb = lds_row_bytes(b.dtype, b.W);
+                const int rb = lds_row_bytes(b.dtype, b.W);  // the block's widest row
                 const int want = p->tune.chunk_rows ? (int)p->tune.chunk_rows : kChunkRows;
                 b.ccr = std::max(2, std::min({8, want, budget / (2 * rb)}));
-                st.lds_bytes = std::max(st.lds_bytes, 2 * b.ccr * rb);
+                // rows of a chunk sit one after another in its buffer, each
+                // at its own width; the buffer is the widest chunk's rows
+                int place = 0, widest = 0;
+                for (int i = 0; i < b.nrows; ++i) {
+                    if (i % b.ccr == 0) place = 0;
+                    int32_t &info = p->abase[(size_t)b.base_off + 2 * (size_t)i + 1];
+                    const int Ki = info & 1023;
+                    info = Ki | (place << 10);
+                    int32_t *row = &p->rel[(size_t)b.tab_off + (size_t)i * kDB];
+                    for (int j = 0; j < kDB; ++j) row[j] += place;
+                    place += lds_row_bytes(b.dtype, 4 * Ki);
+                    widest = std::max(widest, place);
+                }
+                b.bufsz = widest;
+                st.lds_bytes = std::max(st.lds_bytes, 2 * b.bufsz);
             }
-        }
-    }
-    // table entries of row i carry the row's place in its chunk buffer
-    p->rel = p->rel0;
-    for (const BlockDesc &b : p->blocks) {
-        const int rb = lds_row_bytes(b.dtype, b.W);
-        for (int i = 0; i < b.nrows; ++i) {
-            int32_t *row = &p->rel[(size_t)b.tab_off + (size_t)i * kDB];
-            const int32_t add = (i % b.ccr) * rb;
-            for (int j = 0; j < kDB; ++j) row[j] += add;
         }
     }
 }
@@ -862,7 +876,9 @@ extern "C" int mdsm_plan_lds_bytes(const mdsm_plan *p, uint32_t stage, uint64_t 
             // 16-bit entries, 16 for float32 (padding slots read too)
             const uint64_t per_sample = b.dtype == DT_F32 ? 4 : 2;
             r += rows * (uint64_t)(kWaves * b.rpw) * 256u * per_sample;
-            w += rows * (uint64_t)lds_row_bytes(b.dtype, b.W);
+            for (int i = 0; i < b.nrows; ++i)
+                w += (uint64_t)sg.ntiles *
+                     (uint64_t)lds_row_bytes(b.dtype, 4 * (p->abase[(size_t)b.base_off + 2 * i + 1] & 1023));
         }
     }
     *reads = r;
@@ -885,7 +901,7 @@ extern "C" int mdsm_plan_set_tuning(mdsm_plan *p, const mdsm_tuning *t)
     p->tune = *t;
     apply_tuning(p);
     if (p->uploaded) {
-        // only chunk sizes changed: refresh the block and offset tables in place
+        // only chunk sizes changed: refresh the block, offset and base tables in place
         int cur = 0;
         hipError_t e = hipGetDevice(&cur);
         if (e != hipSuccess) return hip_fail(e, "hipGetDevice");
@@ -894,6 +910,10 @@ extern "C" int mdsm_plan_set_tuning(mdsm_plan *p, const mdsm_tuning *t)
                       hipMemcpyHostToDevice);
         if (e == hipSuccess)
             e = hipMemcpy(p->dev.rel, p->rel.data(), p->rel.size() * sizeof(int32_t),
+                          hipMemcpyHostToDevice);
+        // the rows' places in their chunk buffers (row info) move with the chunk size
+        if (e == hipSuccess)
+            e = hipMemcpy(p->dev.abase, p->abase.data(), p->abase.size() * sizeof(int32_t),
                           hipMemcpyHostToDevice);
         if (e != hipSuccess) return hip_fail(e, "updating the block and offset tables");
     }

@@ -183,10 +183,11 @@ def test_library_reports_no_device_or_a_device():
 def test_plan_lds_bytes_from_shift_table():
     """mdsm_plan_lds_bytes against a count made here from the plan's own
     shift table: blocks of up to 64 trials, each read by 64 trial slots of
-    8 bytes per lane (256 samples, 16-bit entries) per row and tile, and its
-    rows staged once per tile at 8 bytes per window sample, the window being
-    the 256-sample tile plus the block's widest shift spread above each
-    row's 16-aligned base, rounded up to 64 (uint8, windows below 1024)."""
+    8 bytes per lane (256 samples, 16-bit entries) per row and tile, and each
+    row staged once per tile at 8 bytes per window sample, the row's window
+    being the 256-sample tile plus the row's own shift spread over the
+    block's trials above its 16-aligned base, rounded up to 64 (uint8,
+    windows below 1024)."""
     C, S = 256, 40000
     ranges = [(0.0, 0.5, 150, 1), (75.0, 1.0, 70, 2)]
     p = M.Plan.brute(C, 1500.0, -300.0 / C, 64e-6, S, "u8", ranges)
@@ -197,8 +198,9 @@ def test_plan_lds_bytes_from_shift_table():
         for j0 in range(0, sh.shape[0], 64):
             blk = sh[j0:j0 + 64]
             base = blk.min(axis=0) & ~15
-            W = -(-(256 + int((blk.max(axis=0) - base).max())) // 64) * 64
-            assert W <= 1024
+            Wrow = -(-(256 + (blk.max(axis=0) - base)) // 64) * 64
+            assert Wrow.max() <= 1024 and Wrow.min() >= 256
+            assert Wrow.min() < Wrow.max(), "rows of a block should differ in width here"
             reads += C * ntiles * 64 * 256 * 2
-            writes += C * ntiles * 8 * W
+            writes += ntiles * 8 * int(Wrow.sum())
     assert p.lds_bytes(0) == (reads, writes)
