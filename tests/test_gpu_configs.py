@@ -7,7 +7,7 @@ c of the (k-summed) input and s_dc the row's effective shift of channel c --
 a checksum of every sample of every row that moves if any shift, window,
 channel or row is wrong.  uint8 input: exact (each sample is rounded back to
 its integer k-sum first).  float16/float32 input: |gpu - fp64| <= 1e-5 times
-the row's L1 mass sum_c sum_t |x[c][t + s_dc]| (the north-star 1e-5 relative
+the row's L1 mass sum_c sum_t |x[c][t + s_dc]| (the path's 1e-5 relative
 bar, applied per row).  Sampled windows at the start, middle and end of rows
 are compared sample by sample with the FP64 oracle (bit-exact for uint8)."""
 import numpy as np

@@ -2,7 +2,7 @@
 
 Bars: uint8 input -> bit-exact (integer sums below 2^24 are exact in float32
 in any order); float16/float32 input -> normwise relative error
-max|gpu - fp64| / max|fp64| <= 1e-5, with the
+max|gpu - fp64| / max|fp64| <= 1e-5 (the path's stated tolerance), with the
 max-abs error printed per case.
 """
 import numpy as np
