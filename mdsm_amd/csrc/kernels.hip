@@ -865,9 +865,10 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
 // Kernel classes: the dtypes a dedispersion kernel is built for.  Each class
 // is its own kernel with its own register allocation, so the integer path's
 // offset registers do not crowd the float paths: launches of float blocks
-// only take the float kernel (config 3: 20.1 -> 19.6 ms), every other launch
-// the kernel of all dtypes (an integer-only kernel was about 1% slower than it
-// on config 2).
+// only take the float kernel (the two-stage subband plan of 4096 float16
+// channels x 2^19 samples, 4096 DMs: 20.1 -> 19.6 ms), every other launch the
+// kernel of all dtypes (an integer-only kernel was about 1% slower than it on
+// the three-range uint8 plan of 1024 channels x 2^20 samples).
 constexpr int kClsInt = 1, kClsFloat = 2, kClsAll = 3;
 
 template <int kCls>
