@@ -177,6 +177,21 @@ struct ItemCtx {
     int64_t in_stride;
 };
 
+// Butterfly halving step of a wave reduction of 2H values per lane: lanes
+// with bit m set keep the upper H values, the others the lower H, and each
+// adds its partner's (lane ^ m) copy of the half it keeps.
+template <int H>
+__device__ __forceinline__ void bfly_half(double *v, int lane, int m)
+{
+    const bool up = (lane & m) != 0;
+#pragma unroll
+    for (int i = 0; i < H; ++i) {
+        const double keep = up ? v[H + i] : v[i];
+        const double send = up ? v[i] : v[H + i];
+        v[i] = keep + __shfl_xor(send, m, 64);
+    }
+}
+
 // ---- LDS layouts ------------------------------------------------------------
 // Every path stages a chunk of channels into LDS in four phases, so that any
 // per-(channel, DM) offset o is one aligned read per lane that returns the
@@ -521,9 +536,10 @@ __device__ __forceinline__ int bcast16(int v, int k)
 // per-lane offset registers (uint8) or scalar loads (the other dtypes) and
 // issues all of that channel (pair)'s LDS reads before the adds.  `one` is
 // 1.0f, passed at run time so that x*one+acc stays a fused convert-and-add.
-template <int DT, int R>
-__device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDesc &bd, int t0, int wave,
-                                 int lane, char *lds, float one)
+template <int DT, int R, bool kStats>
+__device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDesc &bd,
+                                                 const BlockDesc *bdp, int t0, int wave, int lane,
+                                                 char *lds, float one)
 {
     constexpr bool kInt = DT == DT_U8 || DT == DT_U16;
     constexpr int kEB = Layout<DT>::eb;
@@ -824,6 +840,48 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
     }
     flush();
 
+    // Row statistics of integer full-rate rows (launches with stats on): the
+    // sum and sum of squares of each trial's samples t < stat_n in this tile.
+    // The values are integers below 2^24, so both sums are exact in FP64 in
+    // any order.  A butterfly over the wave leaves value v (trial v's sum for
+    // v < 8, trial v-8's sum of squares for v >= 8) in lanes 4v..4v+3.
+    // (The statistics fields are read here, through the descriptor's
+    // pointer, so that they hold no scalar registers across the chunk loop.)
+    if constexpr (kInt && kStats) {
+        const int tile_i = t0 / kTile;
+        const MDSM_CONST_AS BlockDesc *sq = cptr(bdp);
+        const int stat_tiles = sq->stat_tiles;
+        if (tile_i < stat_tiles) {
+            const int stat_n = sq->stat_n;
+            static_assert(R == 8, "the butterfly reduces 2 x 8 values per lane");
+            double v[2 * R];
+#pragma unroll
+            for (int r = 0; r < R; ++r) {
+                double s1 = 0.0, s2 = 0.0;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const double x = (t0 + 4 * lane + q < stat_n) ? (double)f[r][q] : 0.0;
+                    s1 += x;
+                    s2 += x * x;
+                }
+                v[r] = s1;
+                v[R + r] = s2;
+            }
+            bfly_half<8>(v, lane, 32);
+            bfly_half<4>(v, lane, 16);
+            bfly_half<2>(v, lane, 8);
+            bfly_half<1>(v, lane, 4);
+            v[0] += __shfl_xor(v[0], 2, 64);
+            v[0] += __shfl_xor(v[0], 1, 64);
+            const int vi = lane >> 2;
+            const int j = wave * R + (vi & 7);
+            if ((lane & 3) == 0 && j < bd.ntrials) {
+                MDSM_GLOBAL_AS double *st = gptr_w<double>(a.work + sq->stat_off);
+                st[2 * ((int64_t)j * stat_tiles + tile_i) + (vi >> 3)] = v[0];
+            }
+        }
+    }
+
     float *ob = (bd.dst == BUF_OUT ? a.out : (float *)a.work) + bd.out_off;
     const float kf = (float)bd.k;
 #pragma unroll
@@ -871,7 +929,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
 // the three-range uint8 plan of 1024 channels x 2^20 samples).
 constexpr int kClsInt = 1, kClsFloat = 2, kClsAll = 3;
 
-template <int kCls>
+template <int kCls, bool kStats>
 __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, int wave,
                                                 int lane, char *lds)
 {
@@ -896,7 +954,8 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
     const int gt = (int)min<int64_t>(a.group, sg.ntiles - g * a.group);  // tiles in this group
     const int tile_i = __builtin_amdgcn_readfirstlane((int)(g * a.group + r % gt));
     const int bi = __builtin_amdgcn_readfirstlane(sg.first_block + (int)(r / gt));
-    const BlockDesc bd = *cptr(uniform_ptr(sg.blocks + bi));
+    const BlockDesc *bdp = uniform_ptr(sg.blocks + bi);
+    const BlockDesc bd = *cptr(bdp);
     const JobBufs jb = a.jobs[sg.job];
     ItemCtx ctx{uniform_ptr(sg.rel), uniform_ptr(sg.abase), uniform_ptr(jb.in),
                 uniform_ptr(jb.out), uniform_ptr(jb.work), jb.in_stride};
@@ -904,11 +963,11 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
     // float blocks of at most kDB/2 trials run kRPW/2 trials per wave
     if constexpr ((kCls & kClsInt) != 0) {
         if (bd.dtype == DT_U8) {
-            dedisperse_lds_t<DT_U8, kRPW>(ctx, bd, t0, wave, lane, lds, a.one);
+            dedisperse_lds_t<DT_U8, kRPW, kStats>(ctx, bd, bdp, t0, wave, lane, lds, a.one);
             return;
         }
         if (bd.dtype == DT_U16) {
-            dedisperse_lds_t<DT_U16, kRPW>(ctx, bd, t0, wave, lane, lds, a.one);
+            dedisperse_lds_t<DT_U16, kRPW, kStats>(ctx, bd, bdp, t0, wave, lane, lds, a.one);
             return;
         }
     }
@@ -916,14 +975,14 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
         constexpr int H = kRPW / 2;
         if (bd.rpw == kRPW) {
             if (bd.dtype == DT_F16)
-                dedisperse_lds_t<DT_F16, kRPW>(ctx, bd, t0, wave, lane, lds, a.one);
+                dedisperse_lds_t<DT_F16, kRPW, kStats>(ctx, bd, bdp, t0, wave, lane, lds, a.one);
             else
-                dedisperse_lds_t<DT_F32, kRPW>(ctx, bd, t0, wave, lane, lds, a.one);
+                dedisperse_lds_t<DT_F32, kRPW, kStats>(ctx, bd, bdp, t0, wave, lane, lds, a.one);
         } else {  // integer blocks always run kRPW trials per wave
             if (bd.dtype == DT_F16)
-                dedisperse_lds_t<DT_F16, H>(ctx, bd, t0, wave, lane, lds, a.one);
+                dedisperse_lds_t<DT_F16, H, kStats>(ctx, bd, bdp, t0, wave, lane, lds, a.one);
             else
-                dedisperse_lds_t<DT_F32, H>(ctx, bd, t0, wave, lane, lds, a.one);
+                dedisperse_lds_t<DT_F32, H, kStats>(ctx, bd, bdp, t0, wave, lane, lds, a.one);
         }
     }
 }
@@ -931,7 +990,7 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
 #ifndef MDSM_DEDISP_WAVES
 #define MDSM_DEDISP_WAVES 4
 #endif
-template <int kCls>
+template <int kCls, bool kStats>
 __global__ void __launch_bounds__(kThreads) __attribute__((amdgpu_waves_per_eu(MDSM_DEDISP_WAVES)))
 dedisperse_kernel(DedispArgs a)
 {
@@ -946,7 +1005,7 @@ dedisperse_kernel(DedispArgs a)
         const int64_t q = a.items >> 3, rem = a.items & 7;
         const int64_t xcd = g & 7, loc = g >> 3;
         const int64_t f = xcd * q + (xcd < rem ? xcd : rem) + loc;
-        dedisperse_item<kCls>(a, f, wave, lane, lds_raw);
+        dedisperse_item<kCls, kStats>(a, f, wave, lane, lds_raw);
         return;
     }
     // Persistent mode: workgroups pull items from a device counter until the
@@ -962,7 +1021,7 @@ dedisperse_kernel(DedispArgs a)
         const uint32_t fhi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)fv >> 32));
         const int64_t f = (int64_t)(((uint64_t)fhi << 32) | flo);
         if (f >= a.items) break;
-        dedisperse_item<kCls>(a, f, wave, lane, lds_raw);
+        dedisperse_item<kCls, kStats>(a, f, wave, lane, lds_raw);
     }
 }
 
@@ -983,12 +1042,15 @@ hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s)
     if (lds > (size_t)kLdsMax) return hipErrorInvalidValue;
     const bool ints = (L.dtypes & ((1 << DT_U8) | (1 << DT_U16))) != 0;
     const bool floats = (L.dtypes & ((1 << DT_F16) | (1 << DT_F32))) != 0;
-    const int cls = (floats && !ints) ? kClsFloat : kClsAll;
-    void (*kern)(DedispArgs) =
-        cls == kClsFloat ? dedisperse_kernel<kClsFloat> : dedisperse_kernel<kClsAll>;
+    // launches with row statistics (the server's) take a kernel instance of
+    // their own, so that the statistics epilogue costs the others nothing
+    const int cls = (floats && !ints) ? kClsFloat : (L.stats ? kClsAll + 1 : kClsAll);
+    void (*kern)(DedispArgs) = cls == kClsFloat ? dedisperse_kernel<kClsFloat, false>
+                               : cls == kClsAll ? dedisperse_kernel<kClsAll, false>
+                                                : dedisperse_kernel<kClsAll, true>;
     // the full gfx950 LDS to one workgroup, once per kernel (host threads may
     // launch concurrently; setting the attribute twice is harmless)
-    static std::atomic<bool> lds_attr[4];
+    static std::atomic<bool> lds_attr[5];
     if (!lds_attr[cls].load(std::memory_order_acquire)) {
         hipError_t e = hipFuncSetAttribute((const void *)kern,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax);
@@ -1017,8 +1079,8 @@ hipError_t dedisperse_occupancy(int32_t lds_bytes, int32_t dtypes, DedispOccupan
 {
     const bool ints = (dtypes & ((1 << DT_U8) | (1 << DT_U16))) != 0;
     const bool floats = (dtypes & ((1 << DT_F16) | (1 << DT_F32))) != 0;
-    const void *kern = (floats && !ints) ? (const void *)dedisperse_kernel<kClsFloat>
-                                         : (const void *)dedisperse_kernel<kClsAll>;
+    const void *kern = (floats && !ints) ? (const void *)dedisperse_kernel<kClsFloat, false>
+                                         : (const void *)dedisperse_kernel<kClsAll, false>;
     hipFuncAttributes at{};
     hipError_t e = hipFuncGetAttributes(&at, kern);
     if (e != hipSuccess) return e;
@@ -1453,6 +1515,36 @@ __global__ void __launch_bounds__(256)
     }
 }
 
+// Row statistics from the per-tile (sum, sum of squares) partials the
+// dedispersion launch writes (integer values: the sums are exact in FP64 in
+// any order), finished as row_stats_finish does.  One wave per row.
+__global__ void __launch_bounds__(256)
+    row_partials_kernel(const double2 *part, uint32_t ndms, uint32_t tiles, uint32_t n,
+                        double *mean, double *sigma)
+{
+    const uint32_t row = blockIdx.x * 4u + (threadIdx.x >> 6);
+    const uint32_t lane = threadIdx.x & 63u;
+    if (row >= ndms) return;  // whole waves; nothing below synchronises the workgroup
+    const double2 *p = part + (size_t)row * tiles;
+    double a1 = 0.0, a2 = 0.0;
+    for (uint32_t i = lane; i < tiles; i += 64u) {
+        const double2 v = p[i];
+        a1 += v.x;
+        a2 += v.y;
+    }
+    a1 = wave_sum_f64(a1);
+    a2 = wave_sum_f64(a2);
+    if (lane == 0) {
+        const double m = a1 / (double)n;
+        const double ex2 = a2 / (double)n;
+        const double mm = m * m;
+        double var = ex2 - mm;
+        if (var < 0.0) var = 0.0;
+        mean[row] = m;
+        sigma[row] = sqrt(var);
+    }
+}
+
 // mean = s1/n, sigma = sqrt(s2/n - mean*mean) (each a single IEEE operation;
 // this file is compiled with -ffp-contract=off).
 __global__ void row_stats_finish(uint32_t ndms, uint32_t n, double *mean, double *sigma)
@@ -1741,6 +1833,15 @@ hipError_t launch_boxcar_stream(const float *x, uint32_t ndms, uint32_t n, int64
     dim3 grid((uint32_t)((ne + kBT - 1) / kBT), ndms);
     hipLaunchKernelGGL(boxcar_kernel, grid, dim3(256), 0, s, x, n, stride, bs, wmask, maxw, thr,
                        sqrtw, cands, cap, count, mean, sigma);
+    return hipGetLastError();
+}
+
+hipError_t launch_row_partials(const void *part, uint32_t ndms, uint32_t tiles, uint32_t n,
+                               double *mean, double *sigma, hipStream_t s)
+{
+    if (ndms == 0) return hipSuccess;
+    hipLaunchKernelGGL(row_partials_kernel, dim3((ndms + 3) / 4), dim3(256), 0, s,
+                       (const double2 *)part, ndms, tiles, n, mean, sigma);
     return hipGetLastError();
 }
 

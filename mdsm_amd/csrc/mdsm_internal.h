@@ -69,6 +69,13 @@ struct BlockDesc {
     int32_t flush;            // integer path: rows summed in 16-bit lanes between flushes
     int32_t rpw;              // trials per wave: kRPW, or kRPW/2 for float blocks of <= kDB/2 trials
     int32_t bufsz;            // bytes of one chunk buffer (its rows at their own widths)
+    // Row statistics of a uint8 range at full rate (launches with stats on):
+    // item (trial j, tile i < stat_tiles) writes the FP64 pair (sum, sum of
+    // squares) of its samples t < stat_n to work + stat_off + 16*(j*stat_tiles + i).
+    // stat_tiles = 0: the block writes none.
+    int32_t stat_tiles;
+    int32_t stat_n;
+    int64_t stat_off;
 };
 
 // LDS bytes of one staged row: integer data (uint8/uint16) is held as four
@@ -149,6 +156,12 @@ struct RangeHost {
     mdsm_pass pass{};
     int64_t sb_off = 0;           // subband stage-1 buffer (byte offset)
     int32_t len1 = 0;
+    // Row statistics written by the dedispersion launch (uint8 at full rate,
+    // sums exact in FP64): per DM row stat_tiles (sum, sum of squares) pairs
+    // of its first stat_n samples at work + stat_off.  stat_tiles = 0: none.
+    int64_t stat_off = 0;
+    int32_t stat_tiles = 0;
+    int32_t stat_n = 0;
 };
 
 struct KernelTables {
@@ -170,6 +183,7 @@ struct LaunchSpec {
     int32_t grid = 0;               // persistent mode: workgroups (0 = one per item)
     int32_t group = 1;              // tiles per item group
     int32_t dtypes = 0;             // bit d: blocks of DType d in the launch (0 = any)
+    int32_t stats = 0;              // 1: blocks with stat_tiles > 0 write their row statistics
     int32_t njobs = 0;
     JobBufs jobs[kMaxJobs];
 };
@@ -195,12 +209,16 @@ struct mdsm_plan {
     std::string shape_key;             // autotuner cache key
     void *own_work = nullptr;
     uint64_t work_bytes = 0;
+    uint64_t stat_bytes = 0;           // row-statistics partials after the workspace (ranges' stat_off)
     mdsm_plan_info info{};
 };
 
 namespace mdsm {
 
 int upload_plan(mdsm_plan *p);
+// Row statistics of every uint8 full-rate range over its rows' first n
+// samples, written by launches with LaunchSpec::stats (plan.cpp).
+int plan_row_stats(mdsm_plan *p, uint32_t n);
 
 // Switch the manager instance (initialiseMDSM) to device input: from now on
 // start_processing takes each block's new samples from d_in (rows of
@@ -291,6 +309,10 @@ hipError_t launch_boxcar_stream(const float *x, uint32_t ndms, uint32_t n, int64
                                 const SqrtW &sqrtw, mdsm_cand *cands, uint32_t cap,
                                 uint32_t *count, double *mean, double *sigma, bool stats,
                                 hipStream_t s, bool int_exact = false);
+// mean/sigma of ndms rows of n samples from the dedispersion launch's
+// per-tile partials (tiles (sum, sum of squares) FP64 pairs per row).
+hipError_t launch_row_partials(const void *part, uint32_t ndms, uint32_t tiles, uint32_t n,
+                               double *mean, double *sigma, hipStream_t s);
 hipError_t launch_boxcar(const float *x, uint32_t ndms, uint32_t n, int64_t stride,
                          uint32_t wmask, double thr, const SqrtW &sqrtw, mdsm_cand *cands,
                          uint32_t cap, uint32_t *count, double *mean, double *sigma,

@@ -308,6 +308,52 @@ void finish_info(mdsm_plan *p, double adds, double bytes, double dcs)
 
 int upload_plan(mdsm_plan *p) { return p->uploaded ? MDSM_OK : upload(p); }
 
+// Row statistics written by the dedispersion launch (launches with stats
+// on, the server's): uint8 ranges at full rate, over each DM row's first n
+// samples.  The values are integers (float32 sums below 2^24), so the sums
+// are exact in FP64 in any order while n (255 C)^2 < 2^53; ranges beyond
+// that keep none.  The partials sit after the plan's workspace (stat_bytes
+// more).
+int plan_row_stats(mdsm_plan *p, uint32_t n)
+{
+    if (p->subband || n < 1) return MDSM_ERR_INVALID;
+    const uint32_t C = p->obs.nchans;
+    const double vmax = 255.0 * (double)C;
+    const int64_t base = round_up((int64_t)p->work_bytes, 256);
+    int64_t so = base;
+    for (size_t r = 0; r < p->ranges.size(); ++r) {
+        RangeHost &rh = p->ranges[r];
+        rh.stat_tiles = rh.stat_n = 0;
+        rh.stat_off = 0;
+        if (p->obs.dtype != MDSM_U8 || rh.decim >= 0 || n > rh.info.nout ||
+            vmax >= 16777216.0 || (double)n * vmax * vmax >= 9007199254740992.0)
+            continue;
+        rh.stat_n = (int32_t)n;
+        rh.stat_tiles = (int32_t)((n + kTile - 1) / kTile);
+        rh.stat_off = so;
+        so = round_up(so + (int64_t)rh.info.ndms * rh.stat_tiles * 16, 256);
+    }
+    p->stat_bytes = (uint64_t)(so - base);
+    // a brute-force plan has one segment per range, in range order
+    for (size_t r = 0; r < p->ranges.size() && r < p->segs.size(); ++r) {
+        const RangeHost &rh = p->ranges[r];
+        const SegDesc &sg = p->segs[r];
+        for (int bi = sg.first_block; bi < sg.first_block + sg.nblocks; ++bi) {
+            BlockDesc &b = p->blocks[bi];
+            const int64_t j0 = (b.out_off - (int64_t)rh.info.out_offset) / (int64_t)rh.info.nout;
+            b.stat_tiles = rh.stat_tiles;
+            b.stat_n = rh.stat_n;
+            b.stat_off = rh.stat_off + j0 * rh.stat_tiles * 16;
+        }
+    }
+    if (p->uploaded) {
+        hipError_t e = hipMemcpy(p->dev.blocks, p->blocks.data(),
+                                 p->blocks.size() * sizeof(BlockDesc), hipMemcpyHostToDevice);
+        if (e != hipSuccess) return hip_fail(e, "uploading plan blocks");
+    }
+    return MDSM_OK;
+}
+
 mdsm_tuning default_tuning()
 {
     mdsm_tuning t{};

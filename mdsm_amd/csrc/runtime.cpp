@@ -229,8 +229,8 @@ int beam_transient_layout(Beam *b, uint32_t cap)
     uint64_t o = 0;
     b->off_out = o;
     o = up256(o + pi.out_elems * 4);
-    b->off_work = o;
-    o = up256(o + pi.work_bytes);
+    b->off_work = o;  // the plan's workspace, then its row-statistics partials
+    o = up256(up256(o + pi.work_bytes) + b->plan->stat_bytes);
     b->off_cands = o;
     o = up256(o + (uint64_t)nr * cap * sizeof(mdsm_cand));
     b->off_count = o;
@@ -390,6 +390,7 @@ int launch_batch(mdsm_server *s, Batch *bt)
         L.njobs = (int32_t)bt->jobs.size();
         for (Job *j : bt->jobs)
             L.group = std::max<int32_t>(L.group, (int32_t)j->beam->plan->tune.tile_group);
+        L.stats = 1;  // detecting beams' plans carry row statistics (plan_row_stats)
         for (size_t ji = 0; ji < bt->jobs.size(); ++ji) {
             Job *j = bt->jobs[ji];
             L.jobs[ji] = JobBufs{j->beam->ring, j->out, j->work, (int64_t)(j->beam->M + j->beam->N)};
@@ -461,13 +462,23 @@ int launch_batch(mdsm_server *s, Batch *bt)
                 const bool int_exact =
                     b->obs.dtype == MDSM_U8 && ri.downsample == 1 &&
                     255.0 * (double)C * (double)boxcar_maxw(b->wmask) < 16777216.0;
+                // row statistics: from the dedispersion launch's partials
+                // where the plan has them (uint8 at full rate), else a pass
+                // over the output
+                const RangeHost &rh = b->plan->ranges[r];
+                const bool fused = rh.stat_tiles > 0 && (uint32_t)rh.stat_n == n;
                 {
                     TimedLaunch tl("detect", st);
-                    e = launch_boxcar_stream(j->out + ri.out_offset, ri.ndms, n, ri.nout, bs,
-                                             b->wmask, b->det.snr_thr, sw,
-                                             j->cands + (size_t)r * cap, cap, j->count + r,
-                                             j->mean + ri.first_dm, j->sigma + ri.first_dm, true,
-                                             st, int_exact);
+                    if (fused)
+                        e = launch_row_partials(j->work + rh.stat_off, ri.ndms,
+                                                (uint32_t)rh.stat_tiles, n, j->mean + ri.first_dm,
+                                                j->sigma + ri.first_dm, st);
+                    if (e == hipSuccess)
+                        e = launch_boxcar_stream(j->out + ri.out_offset, ri.ndms, n, ri.nout, bs,
+                                                 b->wmask, b->det.snr_thr, sw,
+                                                 j->cands + (size_t)r * cap, cap, j->count + r,
+                                                 j->mean + ri.first_dm, j->sigma + ri.first_dm,
+                                                 !fused, st, int_exact);
                 }
                 if (e != hipSuccess) return hip_fail(e, "server detect");
                 // the last H samples of each row carry over to the next block
@@ -834,6 +845,14 @@ extern "C" int mdsm_beam_open(mdsm_server *s, const mdsm_obs *obs, const mdsm_ra
     if (rc) {
         delete b;
         return rc;
+    }
+    if (b->wmask) {  // the detector's row statistics come with the dedispersion
+        rc = plan_row_stats(b->plan, b->N);
+        if (rc) {
+            mdsm_plan_destroy(b->plan);
+            delete b;
+            return rc;
+        }
     }
     if (b->wmask) {
         b->H = boxcar_carry_len(b->wmask);

@@ -315,10 +315,16 @@ def test_config5_served_two_beams_at_shape(gpu):
         ref = oracle.boxcar_stream(np.concatenate(outs, axis=1), N, list(widths), 6.0)
         for k in range(2):
             c = mix[(b.id, k)][1]
-            got = sorted(zip(c["dm"].tolist(), c["t"].tolist(), c["width"].tolist()))
+            got = sorted(zip(c["dm"].tolist(), c["t"].tolist(), c["width"].tolist(),
+                             c["snr"].tolist()))
             sel = ref[ref["blk"] == k]
-            exp = sorted(zip(sel["dm"].tolist(), sel["t"].tolist(), sel["width"].tolist()))
-            assert got == exp, f"beam {i} block {k}: candidates differ from the oracle"
+            exp = sorted(zip(sel["dm"].tolist(), sel["t"].tolist(), sel["width"].tolist(),
+                             sel["snr"].tolist()))
+            assert [g[:3] for g in got] == [e[:3] for e in exp], \
+                f"beam {i} block {k}: candidates differ from the oracle"
+            # S/N from the row statistics the dedispersion launch writes (exact)
+            assert [g[3] for g in got] == [e[3] for e in exp], \
+                f"beam {i} block {k}: candidate S/N differs from the oracle"
         assert any(t < 0 for t in mix[(b.id, 1)][1]["t"].tolist()), "edge pulse not found"
     # each beam alone: identical outputs and candidates
     for i in range(2):

@@ -156,6 +156,11 @@ def test_alone_and_in_mix_identical_and_match_oracle(gpu):
             assert np.array_equal(out_m, ref[k][0]), f"beam {i} block {k}: output != oracle"
             assert [t[:3] for t in _key(c_m)] == [t[:3] for t in _key(ref[k][1])], \
                 f"beam {i} block {k}: candidates != oracle"
+            # S/N uses the row statistics: from the dedispersion launch's
+            # partials for the full-rate range, from a pass over the output
+            # for the decimated one; both exact, so the S/N is the oracle's
+            assert [t[3] for t in _key(c_m)] == [t[3] for t in _key(ref[k][1])], \
+                f"beam {i} block {k}: candidate S/N != oracle"
     # windows that start in the previous block were searched (t < 0)
     assert any(int(t) < 0 for v in mix.values() for t in v[1]["t"])
     st = srv.stats()
