@@ -1715,6 +1715,13 @@ __global__ void __launch_bounds__(256)
             s[j] = (t >= 0 && t < ne) ? (T)(t < H ? crow[t] : row[t - H]) : (T)0;
         }
     }
+    // slots j of this lane whose position u = kBL*lane + j is one the wave
+    // reports, u in [4, 4 + kBR)
+    uint32_t vm;
+    {
+        const int lo = max(0, 4 - kBL * lane), hi = min(kBL, 4 + kBR - kBL * lane);
+        vm = hi > lo ? ((1u << hi) - 1u) & ~((1u << lo) - 1u) : 0u;
+    }
 #pragma unroll
     for (int lv = 0; lv < 7; ++lv) {
         const uint32_t w = 1u << lv;
@@ -1729,27 +1736,27 @@ __global__ void __launch_bounds__(256)
             const int64_t last = ne - w;
             const int64_t tmax = last - bs.tail;
             const int64_t tmin = H > (int64_t)w ? H - w : 0;
-            bool c[kBL];
-            bool any = false;
-#pragma unroll
-            for (int j = 0; j < kBL; ++j) {
-                const int u = kBL * lane + j;
-                const int64_t t = b + u;
-                bool pass;
+            auto pass = [&](T v) {
                 if constexpr (kF32)
-                    pass = (s[j] - wm32) > lo32;
+                    return (v - wm32) > lo32;
                 else
-                    pass = (s[j] - wm) > lo;
-                c[j] = u >= 4 && u < 4 + kBR && t <= tmax && t >= tmin && pass;
-                any = any || c[j];
-            }
-            if (__any(any)) {
+                    return (v - wm) > lo;
+            };
+            // Quick rejection: the pre-filter is monotone in the sum (a
+            // correctly rounded subtraction is), so no slot of the lane passes
+            // when its largest sum does not.  Slots outside the reported
+            // positions take part here and only cost the exact pass below.
+            T m = s[0];
+#pragma unroll
+            for (int j = 1; j < kBL; ++j) m = s[j] > m ? s[j] : m;
+            if (__any(pass(m))) {
                 const double prevL = (double)shfl_up_d(s[kBL - 1], 1);
                 const double nextF = (double)next_lane_d(s[0]);
 #pragma unroll
                 for (int j = 0; j < kBL; ++j) {
-                    if (!c[j]) continue;
                     const int64_t t = b + kBL * lane + j;
+                    if (!(((vm >> j) & 1u) != 0u && t <= tmax && t >= tmin && pass(s[j])))
+                        continue;
                     const double sv = (double)s[j];
                     const double snr = (sv - wm) / den;
                     if (!(snr > thr)) continue;
