@@ -25,6 +25,7 @@ import argparse
 import collections
 import glob
 import os
+import shlex
 import sqlite3
 import sys
 
@@ -146,7 +147,8 @@ def summarize(args) -> None:
     print(f"rocprofv3 --pmc FETCH_SIZE -d {args.dir}/cfetch -o run -- python3 -m mdsm_amd.profile copy")
     print(f"rocprofv3 --pmc WRITE_SIZE -d {args.dir}/cwrite -o run -- python3 -m mdsm_amd.profile copy")
     print(f"python3 -m mdsm_amd.profile summarize --dir {args.dir} --workload {args.workload} "
-          f"--kernel {args.kernel}" + (f" --blocks {args.blocks}" if args.blocks else ""))
+          f"--kernel {shlex.quote(args.kernel)}" + (f" --blocks {args.blocks}" if args.blocks else "")
+          + (f" --kt-blocks {args.kt_blocks}" if args.kt_blocks else ""))
     print("```\n")
     print("## Kernel statistics (--kernel-trace --stats)\n")
     print("| kernel | calls | total ms | average ms | % |")
@@ -207,6 +209,8 @@ def summarize(args) -> None:
         # the served run's dispatches carry different numbers of blocks: per block
         fb = sum(fk) * 1024.0 * f_scale / args.blocks
         wb = sum(wk) * 1024.0 * w_scale / args.blocks
+        print(f"* counted: the dispatches of `{args.kernel}` only (the served launches; the "
+              f"autotuner's trial launches before them take the instance without row statistics)")
         print(f"* over the {args.blocks} blocks of the counter runs: {fb / 1e9:.4f} GB read and "
               f"{wb / 1e9:.4f} GB written per block")
         if d:
