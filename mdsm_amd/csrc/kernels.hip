@@ -382,8 +382,8 @@ struct RowWin {
 };
 
 // Windows of the rows a wave stages in a chunk: a vector load (a scalar one
-// would tie the prefetch to the LDS waits).  The uint8 path loads them a
-// chunk ahead of their use, through a global-address-space view; the other
+// would tie the prefetch to the LDS waits).  The integer paths load them a
+// chunk ahead of their use, through a global-address-space view; the float
 // paths load them just before the prefetch through a generic pointer, whose
 // flat load waits for both counters, the scalar offsets of the chunk's first
 // row (pair) included (measured faster for them).
@@ -533,7 +533,7 @@ __device__ __forceinline__ int bcast16(int v, int k)
 // Chunks of ccr <= 8 channels are double-buffered in LDS; the next chunk's
 // global loads are in flight (in registers) while the current chunk is
 // summed.  Each wave takes its kRPW trials' offsets of a channel (pair) from
-// per-lane offset registers (uint8) or scalar loads (the other dtypes) and
+// per-lane offset registers (integer rows) or scalar loads (float rows) and
 // issues all of that channel (pair)'s LDS reads before the adds.  `one` is
 // 1.0f, passed at run time so that x*one+acc stays a fused convert-and-add.
 template <int DT, int R, bool kStats>
@@ -767,10 +767,13 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         }
     };
 
-    // uint8 rows take the per-lane offset registers; uint16 and the float
-    // paths, whose wider prefetch or float32 accumulators leave no VGPRs to
-    // spare at 4 waves per SIMD, the scalar offsets
-    constexpr bool kVOff = DT == DT_U8;
+    // integer rows (uint8, uint16 decimated sums) take the per-lane offset
+    // registers; the float paths, whose float32 accumulators leave no VGPRs
+    // to spare at 4 waves per SIMD, the scalar offsets.  (uint16 rows on
+    // scalar offsets: the three-range uint8 plan of 1024 channels x 2^20
+    // samples 48.7 ms against 46.9 with the registers, which spill 4 VGPRs
+    // of the uint16 path.)
+    constexpr bool kVOff = DT == DT_U8 || DT == DT_U16;
     int acc_rows = 0;
     PreRegs<DT> pre;
     if constexpr (kVOff)
@@ -794,15 +797,15 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         const bool more = ch + 1 < nch;
         const int c1 = c0 + ccr;
         const int cc1 = more ? min(ccr, bd.nrows - c1) : 0;
-        // uint8: every vector load issued before this point (this chunk's
+        // integer rows: every vector load issued before this point (this chunk's
         // offset registers, the window starts) has landed by now; waiting
         // here, on every path, keeps the trial loop's reads of the offset
         // registers from waiting for the prefetch below (the counter is in
         // order).
         if constexpr (kVOff) __builtin_amdgcn_s_waitcnt(0x0f70);  // vmcnt(0)
         // the windows of the rows the prefetch below stages, kept (wave-
-        // uniform, in scalar registers) for their staging stores: the uint8
-        // path reloads cb for the chunk after before those stores
+        // uniform, in scalar registers) for their staging stores: the integer
+        // paths reload cb for the chunk after before those stores
         RowWin cs[kPfRows];
         if (more) {  // next chunk's global loads overlap this chunk's compute
             if constexpr (!kVOff) load_colbase<false>(cb, ab + 2 * c1, cc1, wave);
