@@ -600,7 +600,10 @@ int mdsm_beam_push_device(mdsm_server *srv, uint32_t beam_id, const void *d_bloc
  * staging block ([nchans][N] samples of obs.dtype, row stride N); the caller
  * fills it and queues it with mdsm_beam_commit (arrival_s as for push).
  * MDSM_ERR_AGAIN when no block is free.  A lent block counts against the
- * beam's queue depth until committed.  (The reference's caller writes the
+ * beam's queue depth until committed.  Closing or cancelling the beam frees
+ * its staging blocks, lent ones included; a commit while the beam is closing
+ * or exporting its state is refused (MDSM_ERR_STATE) and the block returned
+ * to the beam.  (The reference's caller writes the
  * library-owned input buffer in place the same way,
  * src/pelican-mdsm/modules/MdsmModule.cpp:135-151.) */
 int mdsm_beam_acquire(mdsm_server *srv, uint32_t beam_id, void **host_block);
