@@ -536,7 +536,7 @@ __device__ __forceinline__ int bcast16(int v, int k)
 // per-lane offset registers (integer rows) or scalar loads (float rows) and
 // issues all of that channel (pair)'s LDS reads before the adds.  `one` is
 // 1.0f, passed at run time so that x*one+acc stays a fused convert-and-add.
-template <int DT, int R, bool kStats>
+template <int DT, int R, bool kStats, bool kFloatCls>
 __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDesc &bd,
                                                  const BlockDesc *bdp, int t0, int wave, int lane,
                                                  char *lds, float one)
@@ -768,12 +768,16 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
     };
 
     // integer rows (uint8, uint16 decimated sums) take the per-lane offset
-    // registers; the float paths, whose float32 accumulators leave no VGPRs
-    // to spare at 4 waves per SIMD, the scalar offsets.  (uint16 rows on
-    // scalar offsets: the three-range uint8 plan of 1024 channels x 2^20
-    // samples 48.7 ms against 46.9 with the registers, which spill 4 VGPRs
-    // of the uint16 path.)
-    constexpr bool kVOff = DT == DT_U8 || DT == DT_U16;
+    // registers, and so do float16 rows in the kernel built for float data
+    // alone; float32 rows, and float16 rows in the kernel of all dtypes (whose
+    // register allocation the integer paths set), the scalar offsets.
+    // (uint16 rows on scalar offsets: the three-range uint8 plan of 1024
+    // channels x 2^20 samples 48.7 ms against 46.9 with the registers, which
+    // spill 4 VGPRs of the uint16 path; float16 rows on them in the kernel of
+    // all dtypes cost the served 1024-channel uint8 block 0.7%, in the float
+    // kernel they save the two-stage subband plan of 4096 float16 channels
+    // 0.8%.)
+    constexpr bool kVOff = DT == DT_U8 || DT == DT_U16 || (DT == DT_F16 && kFloatCls);
     int acc_rows = 0;
     PreRegs<DT> pre;
     if constexpr (kVOff)
@@ -963,14 +967,17 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
     ItemCtx ctx{uniform_ptr(sg.rel), uniform_ptr(sg.abase), uniform_ptr(jb.in),
                 uniform_ptr(jb.out), uniform_ptr(jb.work), jb.in_stride};
     const int t0 = tile_i * kTile;
+    constexpr bool kFloat = kCls == kClsFloat;  // the kernel built for float data alone
     // float blocks of at most kDB/2 trials run kRPW/2 trials per wave
     if constexpr ((kCls & kClsInt) != 0) {
         if (bd.dtype == DT_U8) {
-            dedisperse_lds_t<DT_U8, kRPW, kStats>(ctx, bd, bdp, t0, wave, lane, lds, a.one);
+            dedisperse_lds_t<DT_U8, kRPW, kStats, kFloat>(ctx, bd, bdp, t0, wave, lane, lds,
+                                                          a.one);
             return;
         }
         if (bd.dtype == DT_U16) {
-            dedisperse_lds_t<DT_U16, kRPW, kStats>(ctx, bd, bdp, t0, wave, lane, lds, a.one);
+            dedisperse_lds_t<DT_U16, kRPW, kStats, kFloat>(ctx, bd, bdp, t0, wave, lane, lds,
+                                                           a.one);
             return;
         }
     }
@@ -978,14 +985,18 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
         constexpr int H = kRPW / 2;
         if (bd.rpw == kRPW) {
             if (bd.dtype == DT_F16)
-                dedisperse_lds_t<DT_F16, kRPW, kStats>(ctx, bd, bdp, t0, wave, lane, lds, a.one);
+                dedisperse_lds_t<DT_F16, kRPW, kStats, kFloat>(ctx, bd, bdp, t0, wave, lane, lds,
+                                                               a.one);
             else
-                dedisperse_lds_t<DT_F32, kRPW, kStats>(ctx, bd, bdp, t0, wave, lane, lds, a.one);
+                dedisperse_lds_t<DT_F32, kRPW, kStats, kFloat>(ctx, bd, bdp, t0, wave, lane, lds,
+                                                               a.one);
         } else {  // integer blocks always run kRPW trials per wave
             if (bd.dtype == DT_F16)
-                dedisperse_lds_t<DT_F16, H, kStats>(ctx, bd, bdp, t0, wave, lane, lds, a.one);
+                dedisperse_lds_t<DT_F16, H, kStats, kFloat>(ctx, bd, bdp, t0, wave, lane, lds,
+                                                            a.one);
             else
-                dedisperse_lds_t<DT_F32, H, kStats>(ctx, bd, bdp, t0, wave, lane, lds, a.one);
+                dedisperse_lds_t<DT_F32, H, kStats, kFloat>(ctx, bd, bdp, t0, wave, lane, lds,
+                                                            a.one);
         }
     }
 }
