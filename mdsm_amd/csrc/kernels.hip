@@ -1965,6 +1965,8 @@ __global__ void brute_stats_finish(const double *tot, const double *tot2, uint32
     stats[1] = (double)(float)sqrt(m2 / (double)iters - (double)lm2);
 }
 
+constexpr uint32_t kM5 = 16;  // median-of-five outputs per thread
+
 // mode 0: statistics over the whole block (stats[0..1]);
 // mode 1: per-row statistics from the row sums (stats[2..2+2*ndms)), as the
 //         reference's subband output path (dedispersion_output.cpp:86-96);
@@ -1976,9 +1978,9 @@ __global__ void __launch_bounds__(256)
                    mdsm_cand *cands, uint32_t cap, uint32_t *count, const double *stats,
                    uint32_t ndms)
 {
-    // a thread takes 4 consecutive outputs l0..l0+3 from samples l0..l0+7
+    // a thread takes kM5 consecutive outputs l0.. from samples l0..l0+kM5+3
     const uint32_t d = blockIdx.y;
-    const uint32_t l0 = 4u * (blockIdx.x * 256u + threadIdx.x);
+    const uint32_t l0 = kM5 * (blockIdx.x * 256u + threadIdx.x);
     if (n < 5 || l0 >= n - 4) return;
     if (!(dmv[d] > 1.0f)) return;
     float rm = 0.0f, rr = 0.0f;  // row statistics (modes 0 and 1)
@@ -1991,17 +1993,22 @@ __global__ void __launch_bounds__(256)
         rr = (float)sqrt(v);
     }
     const float *r = x + (int64_t)d * stride + l0;
-    float v[8];
-    if (l0 + 8u <= n && ((uintptr_t)r % 16) == 0) {
-        const float4 a = ((const float4 *)r)[0], b = ((const float4 *)r)[1];
-        v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
-        v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    float v[kM5 + 4];
+    if (l0 + kM5 + 4u <= n && ((uintptr_t)r % 16) == 0) {
+#pragma unroll
+        for (int q = 0; q < kM5 / 4 + 1; ++q) {
+            const float4 a = ((const float4 *)r)[q];
+            v[4 * q] = a.x;
+            v[4 * q + 1] = a.y;
+            v[4 * q + 2] = a.z;
+            v[4 * q + 3] = a.w;
+        }
     } else {
 #pragma unroll
-        for (int i = 0; i < 8; ++i) v[i] = l0 + i < n ? r[i] : 0.0f;
+        for (int i = 0; i < kM5 + 4; ++i) v[i] = l0 + i < n ? r[i] : 0.0f;
     }
 #pragma unroll
-    for (int q = 0; q < 4; ++q) {
+    for (int q = 0; q < kM5; ++q) {
         const uint32_t l = l0 + q;
         if (l >= n - 4) break;
         float lm = rm, lr = rr;
@@ -2044,7 +2051,7 @@ hipError_t launch_median5(const float *x, uint32_t ndms, uint32_t n, int64_t str
                            stride, mf, iters, t1, t2);
         hipLaunchKernelGGL(brute_stats_finish, dim3(1), dim3(64), 0, s, t1, t2, iters, mf, stats);
     }
-    dim3 grid(((n - 4 + 3) / 4 + 255) / 256, ndms);
+    dim3 grid(((n - 4 + kM5 - 1) / kM5 + 255) / 256, ndms);
     hipLaunchKernelGGL(median5_kernel, grid, dim3(256), 0, s, x, n, stride, dmv, (float)thr, mode,
                        cmean, crms, spc, cands, cap, count, stats, ndms);
     return hipGetLastError();

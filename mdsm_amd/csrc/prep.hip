@@ -317,10 +317,70 @@ __global__ void __launch_bounds__(256)
 }
 
 // Segments of at most 1024 samples: one wave per segment, the samples held
-// in registers (16 per lane, lane-strided so every load and store is 256
-// contiguous bytes per wave), FP64 sums by wave reductions: the segment is
-// read once and written once.
+// in registers (16 per lane), FP64 sums by wave reductions: the segment is
+// read once and written once.  Whole 1024-sample segments of 16-byte aligned
+// rows take 16 consecutive samples per lane (four 16-byte loads and stores);
+// other segments are lane-strided (256 contiguous bytes per wave and
+// instruction).  The normalisation multiplies by the segment's 1/rms.
 constexpr int kDtK = 16;  // samples per lane
+
+template <bool kVec>
+__device__ __forceinline__ void detrend_wave(float *p, uint32_t L, int lane)
+{
+    const double xc = -(double)L / 2.0 + 0.5;  // abscissa of sample 0
+    auto idx = [&](int k) -> uint32_t {
+        return kVec ? (uint32_t)(kDtK * lane + k) : (uint32_t)lane + 64u * (uint32_t)k;
+    };
+    float y[kDtK];
+    if constexpr (kVec) {
+#pragma unroll
+        for (int q = 0; q < kDtK / 4; ++q) {
+            const float4 v = ((const float4 *)(p + kDtK * lane))[q];
+            y[4 * q] = v.x;
+            y[4 * q + 1] = v.y;
+            y[4 * q + 2] = v.z;
+            y[4 * q + 3] = v.w;
+        }
+    } else {
+#pragma unroll
+        for (int k = 0; k < kDtK; ++k) y[k] = idx(k) < L ? p[idx(k)] : 0.0f;
+    }
+    double sy = 0.0, sxy = 0.0;
+#pragma unroll
+    for (int k = 0; k < kDtK; ++k) {
+        const double v = y[k];
+        sy += v;
+        sxy += (xc + (double)idx(k)) * v;  // zero where the sample is past L
+    }
+    sy = wave_sum_d(sy);
+    sxy = wave_sum_d(sxy);
+    const double sxx = (double)L * ((double)L * (double)L - 1.0) / 12.0;
+    const double a = sy / (double)L;
+    const double b = sxx > 0.0 ? sxy / sxx : 0.0;
+    double ss = 0.0;
+#pragma unroll
+    for (int k = 0; k < kDtK; ++k) {
+        const double r = (double)y[k] - (a + b * (xc + (double)idx(k)));
+        if (idx(k) < L) ss += r * r;
+    }
+    ss = wave_sum_d(ss);
+    const double rms = sqrt(ss / (double)L);
+    const double inv = rms > 0.0 ? 1.0 / rms : 0.0;
+    float o[kDtK];
+#pragma unroll
+    for (int k = 0; k < kDtK; ++k)
+        o[k] = (float)(((double)y[k] - (a + b * (xc + (double)idx(k)))) * inv);
+    if constexpr (kVec) {
+#pragma unroll
+        for (int q = 0; q < kDtK / 4; ++q)
+            ((float4 *)(p + kDtK * lane))[q] =
+                make_float4(o[4 * q], o[4 * q + 1], o[4 * q + 2], o[4 * q + 3]);
+    } else {
+#pragma unroll
+        for (int k = 0; k < kDtK; ++k)
+            if (idx(k) < L) p[idx(k)] = o[k];
+    }
+}
 
 __global__ void __launch_bounds__(256)
     detrend_wave_kernel(float *x, int64_t stride, uint32_t n, uint32_t seg_len, uint32_t nseg)
@@ -332,39 +392,10 @@ __global__ void __launch_bounds__(256)
     const uint32_t s0 = seg * seg_len;
     const uint32_t L = min(seg_len, n - s0);
     float *p = x + (int64_t)row * stride + s0;
-    const double xc = -(double)L / 2.0 + 0.5;  // abscissa of sample 0
-    float y[kDtK];
-    double sy = 0.0, sxy = 0.0;
-#pragma unroll
-    for (int k = 0; k < kDtK; ++k) {
-        const uint32_t i = (uint32_t)lane + 64u * k;
-        y[k] = i < L ? p[i] : 0.0f;
-        const double v = y[k];
-        sy += v;
-        sxy += (xc + (double)i) * v;  // zero where i >= L
-    }
-    sy = wave_sum_d(sy);
-    sxy = wave_sum_d(sxy);
-    const double sxx = (double)L * ((double)L * (double)L - 1.0) / 12.0;
-    const double a = sy / (double)L;
-    const double b = sxx > 0.0 ? sxy / sxx : 0.0;
-    double ss = 0.0;
-#pragma unroll
-    for (int k = 0; k < kDtK; ++k) {
-        const uint32_t i = (uint32_t)lane + 64u * k;
-        const double r = (double)y[k] - (a + b * (xc + (double)i));
-        if (i < L) ss += r * r;
-    }
-    ss = wave_sum_d(ss);
-    const double rms = sqrt(ss / (double)L);
-#pragma unroll
-    for (int k = 0; k < kDtK; ++k) {
-        const uint32_t i = (uint32_t)lane + 64u * k;
-        if (i < L) {
-            const double r = (double)y[k] - (a + b * (xc + (double)i));
-            p[i] = rms > 0.0 ? (float)(r / rms) : 0.0f;
-        }
-    }
+    if (L == 64u * kDtK && ((uintptr_t)p & 15u) == 0)
+        detrend_wave<true>(p, L, lane);
+    else
+        detrend_wave<false>(p, L, lane);
 }
 
 }  // namespace

@@ -78,12 +78,16 @@ def test_corner_turn_full_config2_block(gpu):
     assert np.array_equal(got.sum(axis=1, dtype=np.uint64), x.sum(axis=0, dtype=np.uint64)[::-1])
 
 
-@pytest.mark.parametrize("n,seg", [(4096, 1024), (5000, 1024), (777, 64), (3001, 1000), (5000, 2000)])
-def test_detrend_matches_oracle(gpu, n, seg):
+@pytest.mark.parametrize("n,seg,pad", [(4096, 1024, 5), (5000, 1024, 5), (777, 64, 5),
+                                       (3001, 1000, 5), (5000, 2000, 5),
+                                       (8192, 1024, 0), (5000, 1024, 4)])
+def test_detrend_matches_oracle(gpu, n, seg, pad):
+    """pad 0 or 4: 16-byte aligned rows, whose whole 1024-sample segments
+    take the kernel's 16-consecutive-samples-per-lane form."""
     M = gpu
-    print(f"case detrend n={n} seg={seg}", flush=True)
+    print(f"case detrend n={n} seg={seg} pad={pad}", flush=True)
     rng = np.random.default_rng(13)
-    R, stride = 9, n + 5
+    R, stride = 9, n + pad
     t = np.arange(n)
     x = np.zeros((R, stride), np.float32)
     for r in range(R):
@@ -98,7 +102,7 @@ def test_detrend_matches_oracle(gpu, n, seg):
     print(f"detrend max-abs {ab:.3e} rel {err:.3e}")
     assert err <= 1e-6
     assert np.all(got[3, :n] == 0.0)
-    assert np.array_equal(got[:, n:], x[:, n:])   # padding untouched
+    assert np.array_equal(got[:, n:], x[:, n:])   # padding untouched (if any)
 
 
 @pytest.mark.parametrize("aligned", [False, True])
