@@ -1374,6 +1374,8 @@ __device__ int hist_rank(const uint32_t *hist, uint32_t rank, int lane)
 // RFI clipper (uint8): one workgroup per channel.
 // ---------------------------------------------------------------------------
 
+constexpr int kClipRegs = 16;  // 16-byte words per thread kept in registers (65536 samples a row)
+
 __global__ void __launch_bounds__(256)
     clip_u8_kernel(const uint8_t *in, int64_t in_stride, int64_t t0, uint32_t n, float nsigma,
                    uint8_t *out, int64_t out_stride, int32_t *med_out, int32_t *mad_out,
@@ -1390,23 +1392,33 @@ __global__ void __launch_bounds__(256)
     const uint32_t rank = (n - 1) / 2;
 
     // 16-byte loads and stores when both rows allow them (uniform per
-    // workgroup); the remaining n % 16 samples one at a time
+    // workgroup); the remaining n % 16 samples one at a time.  The first
+    // kClipRegs 16-byte words of each thread (rows up to 65536 samples: all
+    // of them) stay in registers from the histogram pass to the clip pass,
+    // so the row is read once.
     const bool vec = ((uintptr_t)x % 16 == 0) && ((uintptr_t)o % 16 == 0);
     const uint32_t nv = vec ? n / 16u : 0u;
-    auto for_each = [&](auto &&fn) {  // fn(sample index, value) over [0, n)
-        for (uint32_t q = tid; q < nv; q += 256) {
-            const uint4 w = ((const uint4 *)x)[q];
-            const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-            for (int k = 0; k < 16; ++k) fn(16u * q + k, (int)((ws[k >> 2] >> (8 * (k & 3))) & 0xffu));
-        }
-        for (uint32_t t = 16u * nv + tid; t < n; t += 256) fn(t, (int)x[t]);
-    };
+    uint4 keep[kClipRegs];
 #pragma unroll
     for (int w = 0; w < 4; ++w) whist[w][tid] = 0;
     if (tid == 0) sh_cnt = 0;
     __syncthreads();
-    for_each([&](uint32_t, int v) { atomicAdd(&whist[wv][v], 1u); });
+    auto hist_word = [&](const uint4 &w) {
+        const uint32_t ws[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int k = 0; k < 16; ++k) atomicAdd(&whist[wv][(ws[k >> 2] >> (8 * (k & 3))) & 0xffu], 1u);
+    };
+#pragma unroll
+    for (int i = 0; i < kClipRegs; ++i) {
+        const uint32_t q = tid + 256u * i;
+        keep[i] = make_uint4(0u, 0u, 0u, 0u);
+        if (q < nv) {
+            keep[i] = ((const uint4 *)x)[q];
+            hist_word(keep[i]);
+        }
+    }
+    for (uint32_t q = tid + 256u * kClipRegs; q < nv; q += 256) hist_word(((const uint4 *)x)[q]);
+    for (uint32_t t = 16u * nv + tid; t < n; t += 256) atomicAdd(&whist[wv][x[t]], 1u);
     __syncthreads();
     hist[tid] = (whist[0][tid] + whist[1][tid]) + (whist[2][tid] + whist[3][tid]);
     __syncthreads();
@@ -1433,8 +1445,7 @@ __global__ void __launch_bounds__(256)
     const int mad = sh_mad;
     const float thr = (nsigma * 1.4826f) * (float)mad;
     uint32_t mine = 0;
-    for (uint32_t q = tid; q < nv; q += 256) {
-        const uint4 w = ((const uint4 *)x)[q];
+    auto clip_word = [&](uint4 w, uint32_t q) {
         uint32_t ws[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
         for (int k = 0; k < 16; ++k) {
@@ -1446,7 +1457,13 @@ __global__ void __launch_bounds__(256)
             }
         }
         ((uint4 *)o)[q] = make_uint4(ws[0], ws[1], ws[2], ws[3]);
+    };
+#pragma unroll
+    for (int i = 0; i < kClipRegs; ++i) {
+        const uint32_t q = tid + 256u * i;
+        if (q < nv) clip_word(keep[i], q);
     }
+    for (uint32_t q = tid + 256u * kClipRegs; q < nv; q += 256) clip_word(((const uint4 *)x)[q], q);
     for (uint32_t t = 16u * nv + tid; t < n; t += 256) {
         const int v = x[t];
         const bool clip = (float)abs(v - med) > thr;
