@@ -133,7 +133,8 @@ def serve(M, w, args):
     d_to = M.DeviceBuffer(tp.info.out_elems * 4)
     tuning = tp.autotune(d_tx, S, d_to, reps=2)
     del d_tx, d_to, tx
-    srv = Server(max_inflight=16, max_batch=8)
+    srv = Server(max_inflight=getattr(args, "max_inflight", 16),
+                 max_batch=getattr(args, "max_batch", 8))
     det = dict(clip_nsigma=5.0, snr_thr=6.0, widths=widths, cand_cap=1 << 16)
     beams = [srv.open_beam(C, w["fch1"], w["foff"], w["tsamp"], N, "u8", w["ranges"], **det)
              for _ in range(B)]
@@ -773,6 +774,10 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=30.0,
                     help="cfg5: seconds of Poisson arrivals at the real rate (the compressed "
                          "phase runs a quarter of that)")
+    ap.add_argument("--max-batch", type=int, default=8,
+                    help="cfg5: blocks per dedispersion launch of the server")
+    ap.add_argument("--max-inflight", type=int, default=16,
+                    help="cfg5: blocks on the device at once")
     ap.add_argument("--speedup", type=float, default=20.0,
                     help="cfg5: arrival-time compression of the latency phase")
     args = ap.parse_args()
