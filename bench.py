@@ -769,7 +769,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="cfg5", choices=sorted(WORKLOADS))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
-    ap.add_argument("--cpu-dms", type=int, default=48,
+    ap.add_argument("--cpu-dms", type=int, default=64,
                     help="DM trials of the single-thread CPU sample (the all-cores sample takes 4x)")
     ap.add_argument("--serve-seconds", type=float, default=30.0,
                     help="cfg5: seconds of Poisson arrivals at the real rate (the compressed "
