@@ -134,7 +134,8 @@ def serve(M, w, args):
     tuning = tp.autotune(d_tx, S, d_to, reps=2)
     del d_tx, d_to, tx
     srv = Server(max_inflight=getattr(args, "max_inflight", 16),
-                 max_batch=getattr(args, "max_batch", 8))
+                 max_batch=getattr(args, "max_batch", 8),
+                 persistent=getattr(args, "persistent", 0))
     det = dict(clip_nsigma=5.0, snr_thr=6.0, widths=widths, cand_cap=1 << 16)
     beams = [srv.open_beam(C, w["fch1"], w["foff"], w["tsamp"], N, "u8", w["ranges"], **det)
              for _ in range(B)]
@@ -778,6 +779,9 @@ def main():
                     help="cfg5: blocks per dedispersion launch of the server")
     ap.add_argument("--max-inflight", type=int, default=16,
                     help="cfg5: blocks on the device at once")
+    ap.add_argument("--persistent", type=int, default=0,
+                    help="cfg5: server launches on a persistent grid (1: 4 per CU, n > 1: n "
+                         "workgroups; 0: one workgroup per item)")
     ap.add_argument("--speedup", type=float, default=20.0,
                     help="cfg5: arrival-time compression of the latency phase")
     args = ap.parse_args()
