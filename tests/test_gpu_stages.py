@@ -177,7 +177,7 @@ def _np_median5(x, dmv, thr):
     return res, lm, lr
 
 
-@pytest.mark.parametrize("n,stride", [(4000, 4000), (4001, 4003), (4003, 4004)])
+@pytest.mark.parametrize("n,stride", [(4000, 4000), (4001, 4003), (4003, 4004), (4005, 4008)])
 def test_median5_detector(gpu, n, stride):
     """Aligned rows (16-byte loads) and unaligned or ragged ones."""
     M = gpu
