@@ -58,7 +58,8 @@ def build(verbose: bool = False) -> str:
 
 
 DRIVER_SRC = os.path.join(HERE, "driver", "mdsm_file.cpp")
-DRIVER = os.path.join(HERE, "mdsm_file")
+# the driver sits next to the library it links (found through $ORIGIN)
+DRIVER = os.path.join(os.path.dirname(LIB), "mdsm_file")
 
 
 def build_driver(verbose: bool = False) -> str:
