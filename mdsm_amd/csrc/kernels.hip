@@ -193,13 +193,25 @@ __device__ __forceinline__ void bfly_half(double *v, int lane, int m)
 }
 
 // ---- LDS layouts ------------------------------------------------------------
-// Every path stages a chunk of channels into LDS in four phases, so that any
-// per-(channel, DM) offset o is one aligned read per lane that returns the
-// lane's 4 consecutive samples (phase o&3, entry (o>>2) + lane):
-//  * uint8 / uint16 / float16: entry k of phase s packs samples u = 4k+s+{0..3}
-//    as 16-bit lanes, .x = x[u] | x[u+2] << 16, .y = x[u+1] | x[u+3] << 16
-//    (8 bytes, ds_read_b64);
-//  * float32: entry k of phase s holds x[u..u+3] (16 bytes, ds_read_b128).
+// Every path stages a chunk of channels into LDS in phase-shifted copies, so
+// that any per-(channel, DM) offset o is one read per lane of aligned
+// addresses that returns the lane's 4 samples:
+//  * uint8 / uint16: four phases of 8-byte entries; entry k of phase s packs
+//    samples u = 4k+s+{0..3} as 16-bit lanes, .x = x[u] | x[u+2] << 16,
+//    .y = x[u+1] | x[u+3] << 16 (one ds_read_b64 at entry (o>>2) + L of
+//    phase o&3: lane L holds samples 4L..4L+3 of the tile);
+//  * float16: two planes of 16-bit pairs, dword j of plane p =
+//    x[2j+p] | x[2j+p+1] << 16; offset o reads dwords (o>>1) + L and
+//    (o>>1) + L + 64 of plane o&1 (one ds_read2st64_b32): lane L holds
+//    samples 2L, 2L+1, 128+2L, 129+2L.  Half the staged bytes of four phases;
+//    the reads run slower (ds_read2st64_b32 74 TB/s against 85 for
+//    ds_read_b64 at 4 waves per SIMD), which the integer rows, bound by their
+//    reads, do not repay (the three-range uint8 plan of 1024 channels x 2^20
+//    samples 46.9 -> 56.0 ms) and float16 rows, bound by their
+//    v_fma_mix_f32, do.  (An 8-byte ds_read_b64 at a 4-byte-aligned address
+//    runs 16x slower, ds_read2_b32 of adjacent dwords at half the rate.)
+//  * float32: two phases of float2 entries, read like the float16 planes
+//    (ds_read2st64_b64), or four phases of float4 entries (ds_read_b128).
 // Integer samples are summed two channels at a time with one v_add3_u32 per
 // half into 16-bit lanes, flushed to float32 before they can overflow;
 // float16 samples are converted and added in one v_fma_mix_f32 each (x*1+acc,
@@ -253,8 +265,22 @@ template <int DT>
 struct Layout {
     static constexpr int es = DT == DT_U8 ? 1 : (DT == DT_F32 ? 4 : 2);  // bytes per sample
     static constexpr int raw = DT == DT_U8 ? 2 : (DT == DT_F32 ? 8 : 4);  // dwords per 8 samples
-    static constexpr int eb = (DT == DT_F32 && !kF32Two) ? 16 : 8;         // bytes per LDS entry
+    static constexpr bool planes = two_planes16(DT);  // two planes of 16-bit pairs
+    // bytes between neighbouring lanes' reads (the entry or dword size)
+    static constexpr int eb = DT == DT_F32 ? (kF32Two ? 8 : 16) : (planes ? 4 : 8);
+    // lane L holds samples 2L, 2L+1, 128+2L, 129+2L of the tile (else 4L..4L+3)
+    static constexpr bool split = DT == DT_F32 ? kF32Two : planes;
 };
+
+// Tile sample of a lane's accumulator q (0..3).
+template <int DT>
+__device__ __forceinline__ int lane_sample(int lane, int q)
+{
+    if constexpr (Layout<DT>::split)
+        return 128 * (q >> 1) + 2 * lane + (q & 1);
+    else
+        return 4 * lane + q;
+}
 
 // Sample-by-sample fetch of 8 samples at a row end or an odd row start (zero
 // past in_len): out of line, it is rarely taken and would otherwise be
@@ -335,6 +361,32 @@ __device__ __forceinline__ void phases16(const uint32_t *raw, uint32_t (&d0)[4],
         d0[1] = Q0; d1[1] = P1;
         d0[2] = P1; d1[2] = Q1;
         d0[3] = Q1; d1[3] = P2;
+    }
+}
+
+// The two 16-bit planes of entry column k (samples 4k..4k+4 of its raw
+// registers, 16-bit samples): dwords 2k and 2k+1 of plane p are w[p][0],
+// w[p][1].
+__device__ __forceinline__ void planes16(const uint32_t *raw, uint32_t (&w)[2][2])
+{
+    const uint32_t h0 = raw[0], h1 = raw[1], h2 = raw[2];  // samples 0-1, 2-3, 4-5
+    w[0][0] = h0;
+    w[0][1] = h1;
+    w[1][0] = perm(h1, h0, 0x05040302u);
+    w[1][1] = perm(h2, h1, 0x05040302u);
+}
+
+// A lane's read of 4 staged 16-bit samples at p: two dwords 256 bytes apart
+// (one ds_read2st64_b32) in the two-plane layout, one ds_read_b64 of a
+// four-phase entry otherwise.
+template <int DT>
+__device__ __forceinline__ uint2 ld16(const char *p)
+{
+    if constexpr (Layout<DT>::planes) {
+        const uint32_t *q = (const uint32_t *)p;
+        return make_uint2(q[0], q[64]);
+    } else {
+        return *(const uint2 *)p;
     }
 }
 
@@ -465,6 +517,23 @@ __device__ __forceinline__ void store_chunk(const PreRegs<DT> &pre, char *buf,
                 if (64 * it >= K) break;
                 if (k < K) store_phases<DT>(row, K, k, pre[rr][it]);
             }
+        } else if constexpr (Layout<DT>::planes) {
+            // the lane's column in both planes (2W = 8K bytes apart),
+            // computed once per row; entry column lane + 64*it sits 512*it
+            // bytes further.  Contiguous 8-byte stores: no bank conflicts.
+            uint2 *q0 = (uint2 *)(row + 8 * lane);
+            uint2 *q1 = (uint2 *)(row + 8 * K + 8 * lane);
+#pragma unroll
+            for (int it = 0; it < pf_iters<DT>(); ++it) {
+                const int k = lane + 64 * it;
+                if (64 * it >= K) break;
+                if (k < K) {
+                    uint32_t w[2][2];
+                    planes16(pre[rr][it], w);
+                    q0[64 * it] = make_uint2(w[0][0], w[0][1]);
+                    q1[64 * it] = make_uint2(w[1][0], w[1][1]);
+                }
+            }
         } else {
             // the lane's entry in each of the four phase planes (8*K bytes
             // apart), computed once per row; entry column lane + 64*it sits
@@ -559,23 +628,27 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
 #pragma unroll
         for (int j = 0; j < 4; ++j) f[r][j] = 0.0f;
     }
+    // accumulators of the 16-bit halves of a read's second (.x hi) and
+    // third (.y lo) samples: 2L+1 and 128+2L (two planes), 4L+2 and 4L+1
+    // (four phases)
+    constexpr int qxh = Layout<DT>::planes ? 1 : 2, qyl = Layout<DT>::planes ? 2 : 1;
     auto flush = [&]() {
         if constexpr (kInt) {
 #pragma unroll
             for (int r = 0; r < R; ++r) {
                 f[r][0] += (float)(ae[r] & 0xffffu);
-                f[r][2] += (float)(ae[r] >> 16);
-                f[r][1] += (float)(ao[r] & 0xffffu);
+                f[r][qxh] += (float)(ae[r] >> 16);
+                f[r][qyl] += (float)(ao[r] & 0xffffu);
                 f[r][3] += (float)(ao[r] >> 16);
                 ae[r] = ao[r] = 0u;
             }
         }
     };
-    auto add_entry = [&](int r, uint2 x) {  // one 16-bit entry into trial r
+    auto add_entry = [&](int r, uint2 x) {  // one 16-bit read into trial r
         if constexpr (DT == DT_F16) {
             f[r][0] = __builtin_fmaf(half_lo(x.x), one, f[r][0]);
-            f[r][2] = __builtin_fmaf(half_hi(x.x), one, f[r][2]);
-            f[r][1] = __builtin_fmaf(half_lo(x.y), one, f[r][1]);
+            f[r][qxh] = __builtin_fmaf(half_hi(x.x), one, f[r][qxh]);
+            f[r][qyl] = __builtin_fmaf(half_lo(x.y), one, f[r][qyl]);
             f[r][3] = __builtin_fmaf(half_hi(x.y), one, f[r][3]);
         } else {
             ae[r] += x.x;
@@ -635,8 +708,8 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 uint2 x[R], y[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    x[r] = *(const uint2 *)(lane_base + oi[r]);
-                    y[r] = *(const uint2 *)(lane_base + oj[r]);
+                    x[r] = ld16<DT>(lane_base + oi[r]);
+                    y[r] = ld16<DT>(lane_base + oj[r]);
                 }
                 __builtin_amdgcn_sched_barrier(0);
                 {
@@ -662,7 +735,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
             if (i < cc) {
                 uint2 x[R];
 #pragma unroll
-                for (int r = 0; r < R; ++r) x[r] = *(const uint2 *)(lane_base + oi[r]);
+                for (int r = 0; r < R; ++r) x[r] = ld16<DT>(lane_base + oi[r]);
                 __builtin_amdgcn_sched_barrier(0);
                 load_soffsets(grow + cc * kDB);
                 __builtin_amdgcn_sched_barrier(0);
@@ -739,8 +812,8 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 uint2 x[R], y[R];
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
-                    x[r] = *(const uint2 *)(lane_base + bcast16(v0, r));
-                    y[r] = *(const uint2 *)(lane_base + bcast16(v0, 8 + r));
+                    x[r] = ld16<DT>(lane_base + bcast16(v0, r));
+                    y[r] = ld16<DT>(lane_base + bcast16(v0, 8 + r));
                 }
                 // all 16 reads in flight before the first add waits on one
                 __builtin_amdgcn_sched_barrier(0);
@@ -762,7 +835,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
                 for (int r = 0; r < R; ++r)
-                    add_entry(r, *(const uint2 *)(lane_base + bcast16(v0, r)));
+                    add_entry(r, ld16<DT>(lane_base + bcast16(v0, r)));
             }
         }
     };
@@ -867,7 +940,8 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 double s1 = 0.0, s2 = 0.0;
 #pragma unroll
                 for (int q = 0; q < 4; ++q) {
-                    const double x = (t0 + 4 * lane + q < stat_n) ? (double)f[r][q] : 0.0;
+                    const double x =
+                        (t0 + lane_sample<DT>(lane, q) < stat_n) ? (double)f[r][q] : 0.0;
                     s1 += x;
                     s2 += x * x;
                 }
@@ -900,7 +974,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         for (int q = 0; q < 4; ++q) v[q] = bd.k > 1 ? __fdiv_rn(f[r][q], kf) : f[r][q];
         float *og = ob + (int64_t)j * bd.out_trial_stride;
         MDSM_GLOBAL_AS float *o = gptr_w<float>(og);
-        if constexpr (DT == DT_F32 && kF32Two) {
+        if constexpr (Layout<DT>::split) {
             // samples t0 + 2L + {0, 1} and t0 + 128 + 2L + {0, 1}
             const bool al = (((uintptr_t)(og + t0)) & 7) == 0;
 #pragma unroll

@@ -78,20 +78,27 @@ struct BlockDesc {
     int64_t stat_off;
 };
 
-// LDS bytes of one staged row: integer data (uint8/uint16) is held as four
-// phase-shifted copies of uint16 pairs (8 bytes per sample), float data as
-// float32 (4 bytes per sample).
-// Staged LDS rows: four phases of 16-bit entries (8 bytes per 4 samples) for
-// uint8/uint16/float16; for float32 two phases of float2 entries (8 bytes
-// per 2 samples) or, with MDSM_F32_2PH=0, four phases of float4 entries.
+// Staged LDS rows of W samples:
+//  * uint8 / uint16: four phases of 8-byte entries of 16-bit lanes (8 bytes
+//    per sample; one ds_read_b64 per lane and offset);
+//  * float16: two planes of 16-bit pairs, dword j of plane p holding samples
+//    2j+p and 2j+p+1 (4 bytes per sample; one ds_read2st64_b32);
+//  * float32: two phases of float2 entries (8 bytes per sample) or, with
+//    MDSM_F32_2PH=0, four phases of float4 entries.
 #ifndef MDSM_F32_2PH
 #define MDSM_F32_2PH 1
 #endif
-inline int lds_row_bytes(int dtype, int W) { return (dtype == DT_F32 && !MDSM_F32_2PH) ? 16 * W : 8 * W; }
+constexpr bool two_planes16(int dtype) { return dtype == DT_F16; }
+inline int lds_row_bytes(int dtype, int W)
+{
+    if (dtype == DT_F32) return MDSM_F32_2PH ? 8 * W : 16 * W;
+    return two_planes16(dtype) ? 4 * W : 8 * W;
+}
 // Byte offset of shift o (relative to the row base) in a staged row of W samples.
 inline int32_t lds_offset(int dtype, int W, int32_t o)
 {
     const int K = W / 4;
+    if (two_planes16(dtype)) return (o & 1) * (2 * W) + (o >> 1) * 4;
     if (dtype != DT_F32) return (o & 3) * K * 8 + (o >> 2) * 8;
     if (MDSM_F32_2PH) return (o & 1) * (2 * K) * 8 + (o >> 1) * 8;
     return (o & 3) * K * 16 + (o >> 2) * 16;
