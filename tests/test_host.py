@@ -204,3 +204,25 @@ def test_plan_lds_bytes_from_shift_table():
             reads += C * ntiles * 64 * 256 * 2
             writes += ntiles * 8 * int(Wrow.sum())
     assert p.lds_bytes(0) == (reads, writes)
+
+
+def test_plan_lds_bytes_float16_two_planes():
+    """The same count for float16 rows, which are staged in two planes of
+    16-bit pairs: 4 bytes of LDS per window sample (half the four-phase
+    layout of integer rows), 8 bytes read per lane and trial slot; a float
+    block of at most 32 trials runs 4 trials per wave (32 slots)."""
+    C, S = 256, 40000
+    p = M.Plan.brute(C, 1500.0, -300.0 / C, 64e-6, S, "f16", [(0.0, 0.5, 150, 1)])
+    sh = p.shift_table(0)
+    ntiles = -(-p.ranges[0].nout // 256)
+    reads = writes = 0
+    for j0 in range(0, sh.shape[0], 64):
+        blk = sh[j0:j0 + 64]
+        base = blk.min(axis=0) & ~15
+        Wrow = -(-(256 + (blk.max(axis=0) - base)) // 64) * 64
+        assert Wrow.max() <= 1024
+        slots = 32 if blk.shape[0] <= 32 else 64
+        reads += C * ntiles * slots * 256 * 2
+        writes += ntiles * 4 * int(Wrow.sum())
+    assert sh.shape[0] % 64 <= 32, "the last block should run 4 trials per wave here"
+    assert p.lds_bytes(0) == (reads, writes)
