@@ -183,6 +183,29 @@ def test_ragged_stride_and_ns_rule(gpu):
     assert np.array_equal(got, exact)
 
 
+def test_ragged_stride_float16(gpu):
+    """float16 rows whose starts are 2 bytes off a dword (odd row stride) and
+    an odd row length: the two-plane staging's per-sample fetch at row starts
+    and ends, odd and even shifts, windows wider than one tile."""
+    M = gpu
+    C, S, stride = 33, 3001, 3333
+    print("case ragged_stride_float16", flush=True)
+    x = M.synth_filterbank(C, S, "f16", seed=6, gain=(0.5, 2.0), stride=stride,
+                           foff=-300.0 / C, pulses=[(40.0, 500, 3, 2.0)])
+    plan = _plan(M, C, S, "f16", [(3.0, 1.7, 45, 1)], foff=-300.0 / C, rule=1)
+    sh = plan.shift_table(0)
+    assert (sh % 2 == 1).any() and (sh % 2 == 0).any() and sh.max() > 256
+    din = M.DeviceBuffer.from_host(x)
+    dout = M.DeviceBuffer(plan.info.out_elems * 4)
+    plan.dedisperse(din, stride, dout)
+    M.synchronize()
+    got = dout.download(np.float32, (plan.info.out_elems,)).reshape(45, -1)
+    ref = oracle.range_f64(np.ascontiguousarray(x[:, :S]), 1, sh, plan.ranges[0].nout)
+    e, a = rel_err(got, ref)
+    print(f"ragged float16 rel {e:.3e} maxabs {a:.3e}")
+    assert e <= TOL
+
+
 def test_config1_full_size(gpu):
     """Config 1 at full size (256 ch, 65536 samples, 512 DMs, f32, pulse at DM 56.8):
     all DM rows compared to the FP64 oracle on a subset of 64 rows."""
