@@ -23,12 +23,7 @@ constexpr int kWaves = MDSM_WAVES;   // waves per workgroup
 constexpr int kRPW = MDSM_RPW;       // DM trials per wave (registers)
 constexpr int kThreads = 64 * kWaves;
 constexpr int kDB = kWaves * kRPW;   // DM trials per block
-#ifndef MDSM_GROUPS
-#define MDSM_GROUPS 1
-#endif
-constexpr int kGroups = MDSM_GROUPS; // integer path: 4-sample groups per lane, 256 apart
-constexpr int kPPL = 4 * kGroups;    // output samples per lane
-constexpr int kTile = 256 * kGroups; // output samples per tile
+constexpr int kTile = 256;          // output samples per tile (4 per lane)
 
 // Buffers a block can read or write.
 enum BufId : int32_t { BUF_IN = 0, BUF_OUT = 1, BUF_WORK = 2 };
