@@ -210,8 +210,10 @@ __device__ __forceinline__ void bfly_half(double *v, int lane, int m)
 //    samples 46.9 -> 56.0 ms) and float16 rows, bound by their
 //    v_fma_mix_f32, do.  (An 8-byte ds_read_b64 at a 4-byte-aligned address
 //    runs 16x slower, ds_read2_b32 of adjacent dwords at half the rate.)
-//  * float32: two phases of float2 entries, read like the float16 planes
-//    (ds_read2st64_b64), or four phases of float4 entries (ds_read_b128).
+//  * float32: two phases of float2 entries, entry j of phase p =
+//    (x[2j+p], x[2j+p+1]), read like the float16 planes (ds_read2st64_b64).
+//    (Four phases of float4 entries read by ds_read_b128 read faster, but
+//    their 16 bytes per staged sample leave wide windows 2 rows per chunk.)
 // Integer samples are summed two channels at a time with one v_add3_u32 per
 // half into 16-bit lanes, flushed to float32 before they can overflow;
 // float16 samples are converted and added in one v_fma_mix_f32 each (x*1+acc,
@@ -253,13 +255,6 @@ __device__ __forceinline__ uint32_t load_u(const void *base, int64_t i)
         return gptr<uint16_t>(base)[i];
 }
 
-#ifndef MDSM_F32_2PH
-#define MDSM_F32_2PH 1
-#endif
-// float32 layout: MDSM_F32_2PH=1 two phases of float2 entries (a lane's 4
-// samples are 2L, 2L+1, 128+2L, 129+2L: one ds_read2st64_b64); 0: four
-// phases of float4 entries (4 consecutive samples, ds_read_b128).
-constexpr bool kF32Two = MDSM_F32_2PH != 0;
 
 template <int DT>
 struct Layout {
@@ -267,9 +262,9 @@ struct Layout {
     static constexpr int raw = DT == DT_U8 ? 2 : (DT == DT_F32 ? 8 : 4);  // dwords per 8 samples
     static constexpr bool planes = two_planes16(DT);  // two planes of 16-bit pairs
     // bytes between neighbouring lanes' reads (the entry or dword size)
-    static constexpr int eb = DT == DT_F32 ? (kF32Two ? 8 : 16) : (planes ? 4 : 8);
+    static constexpr int eb = DT == DT_F32 ? 8 : (planes ? 4 : 8);
     // lane L holds samples 2L, 2L+1, 128+2L, 129+2L of the tile (else 4L..4L+3)
-    static constexpr bool split = DT == DT_F32 ? kF32Two : planes;
+    static constexpr bool split = DT == DT_F32 || planes;
 };
 
 // Tile sample of a lane's accumulator q (0..3).
@@ -390,30 +385,25 @@ __device__ __forceinline__ uint2 ld16(const char *p)
     }
 }
 
-// Write the four phases of entry column k from its 8 raw samples.
-template <int DT>
-__device__ __forceinline__ void store_phases(char *row, int K, int k, const uint32_t *raw)
+// Write entry column k of a float32 row (samples 4k..4k+4 of its raw
+// registers): entries 2k and 2k+1 of both phases (2K entries each).
+__device__ __forceinline__ void store_f32_phases(char *row, int K, int k, const uint32_t *raw)
 {
-    if constexpr (DT == DT_F32 && kF32Two) {
-        // group k covers samples 4k..4k+3: entries 2k, 2k+1 of both phases
-        const int K2 = 2 * K;
-        uint2 *r2 = (uint2 *)row;
-        r2[2 * k] = make_uint2(raw[0], raw[1]);
-        r2[2 * k + 1] = make_uint2(raw[2], raw[3]);
-        r2[K2 + 2 * k] = make_uint2(raw[1], raw[2]);
-        r2[K2 + 2 * k + 1] = make_uint2(raw[3], raw[4]);
-        return;
-    } else if constexpr (DT == DT_F32) {
-#pragma unroll
-        for (int ph = 0; ph < 4; ++ph)
-            ((uint4 *)row)[ph * K + k] = make_uint4(raw[ph], raw[ph + 1], raw[ph + 2], raw[ph + 3]);
-        return;
-    } else {
-        uint32_t d0[4], d1[4];
-        phases16<DT>(raw, d0, d1);
-#pragma unroll
-        for (int ph = 0; ph < 4; ++ph) ((uint2 *)row)[ph * K + k] = make_uint2(d0[ph], d1[ph]);
-    }
+    const int K2 = 2 * K;
+    uint2 *r2 = (uint2 *)row;
+    r2[2 * k] = make_uint2(raw[0], raw[1]);
+    r2[2 * k + 1] = make_uint2(raw[2], raw[3]);
+    r2[K2 + 2 * k] = make_uint2(raw[1], raw[2]);
+    r2[K2 + 2 * k + 1] = make_uint2(raw[3], raw[4]);
+}
+
+// A lane's read of 4 staged float32 samples at p: two float2 entries 512
+// bytes apart (one ds_read2st64_b64), samples 2L, 2L+1, 128+2L, 129+2L.
+__device__ __forceinline__ float4 ld_f32(const char *p)
+{
+    const float2 *q = (const float2 *)p;
+    const float2 lo = q[0], hi = q[64];
+    return make_float4(lo.x, lo.y, hi.x, hi.y);
 }
 
 // Register prefetch of one chunk: wave w owns rows w, w+kWaves, ... of the
@@ -515,7 +505,7 @@ __device__ __forceinline__ void store_chunk(const PreRegs<DT> &pre, char *buf,
             for (int it = 0; it < pf_iters<DT>(); ++it) {
                 const int k = lane + 64 * it;
                 if (64 * it >= K) break;
-                if (k < K) store_phases<DT>(row, K, k, pre[rr][it]);
+                if (k < K) store_f32_phases(row, K, k, pre[rr][it]);
             }
         } else if constexpr (Layout<DT>::planes) {
             // the lane's column in both planes (2W = 8K bytes apart),
@@ -678,15 +668,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
             for (int i = 0; i < cc; ++i) {
                 float4 x[R];
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    if constexpr (kF32Two) {
-                        const float2 *q = (const float2 *)(lane_base + oi[r]);
-                        const float2 lo = q[0], hi = q[64];
-                        x[r] = make_float4(lo.x, lo.y, hi.x, hi.y);
-                    } else {
-                        x[r] = *(const float4 *)(lane_base + oi[r]);
-                    }
-                }
+                for (int r = 0; r < R; ++r) x[r] = ld_f32(lane_base + oi[r]);
                 __builtin_amdgcn_sched_barrier(0);
                 {
                     const cint32_t *rn = as_const(grow + (i + 1) * kDB);
@@ -769,16 +751,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
             auto row = [&](int o_pair, int half) {
                 float4 x[R];
 #pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    const int o = bcast16(o_pair, 8 * half + r);
-                    if constexpr (kF32Two) {
-                        const float2 *q = (const float2 *)(lane_base + o);
-                        const float2 lo = q[0], hi = q[64];
-                        x[r] = make_float4(lo.x, lo.y, hi.x, hi.y);
-                    } else {
-                        x[r] = *(const float4 *)(lane_base + o);
-                    }
-                }
+                for (int r = 0; r < R; ++r) x[r] = ld_f32(lane_base + bcast16(o_pair, 8 * half + r));
 #pragma unroll
                 for (int r = 0; r < R; ++r) {
                     f[r][0] += x[r].x;

@@ -78,25 +78,20 @@ struct BlockDesc {
 //    per sample; one ds_read_b64 per lane and offset);
 //  * float16: two planes of 16-bit pairs, dword j of plane p holding samples
 //    2j+p and 2j+p+1 (4 bytes per sample; one ds_read2st64_b32);
-//  * float32: two phases of float2 entries (8 bytes per sample) or, with
-//    MDSM_F32_2PH=0, four phases of float4 entries.
-#ifndef MDSM_F32_2PH
-#define MDSM_F32_2PH 1
-#endif
+//  * float32: two phases of float2 entries, entry j of phase p holding
+//    samples 2j+p and 2j+p+1 (8 bytes per sample; one ds_read2st64_b64).
 constexpr bool two_planes16(int dtype) { return dtype == DT_F16; }
 inline int lds_row_bytes(int dtype, int W)
 {
-    if (dtype == DT_F32) return MDSM_F32_2PH ? 8 * W : 16 * W;
+    if (dtype == DT_F32) return 8 * W;
     return two_planes16(dtype) ? 4 * W : 8 * W;
 }
 // Byte offset of shift o (relative to the row base) in a staged row of W samples.
 inline int32_t lds_offset(int dtype, int W, int32_t o)
 {
-    const int K = W / 4;
     if (two_planes16(dtype)) return (o & 1) * (2 * W) + (o >> 1) * 4;
-    if (dtype != DT_F32) return (o & 3) * K * 8 + (o >> 2) * 8;
-    if (MDSM_F32_2PH) return (o & 1) * (2 * K) * 8 + (o >> 1) * 8;
-    return (o & 3) * K * 16 + (o >> 2) * 16;
+    if (dtype == DT_F32) return (o & 1) * (4 * W) + (o >> 1) * 8;
+    return (o & 3) * (2 * W) + (o >> 2) * 8;  // four phases of W/4 entries
 }
 inline bool int_path(int dtype) { return dtype == DT_U8 || dtype == DT_U16; }
 // Widest staged row (samples) of a block: bounded by the kernel's register
