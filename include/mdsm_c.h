@@ -754,6 +754,15 @@ int mdsm_probe_lds_read(uint32_t bytes_per_lane, uint32_t waves_per_simd, double
  * kernel's staging stores).  No counterpart in the reference. */
 int mdsm_probe_lds_write(uint32_t bytes_per_lane, uint32_t waves_per_simd, double *tbs);
 
+/* LDS read bandwidth (TB/s) of the read forms the dedispersion kernel's row
+ * layouts chose between, the same way, at waves_per_simd (1..8): form 1 two
+ * dwords 256 bytes apart per lane (ds_read2st64_b32: the float16 rows'
+ * reads), 2 two 8-byte entries 512 bytes apart (ds_read2st64_b64: the
+ * float32 rows'), 3 8 bytes at a 4-byte-aligned, not 8-byte-aligned,
+ * address (ds_read_b64), 4 the same 8 bytes as two adjacent dwords
+ * (ds_read2_b32).  No counterpart in the reference. */
+int mdsm_probe_lds_read_form(uint32_t form, uint32_t waves_per_simd, double *tbs);
+
 const char *mdsm_last_error(void);
 const char *mdsm_version(void);
 

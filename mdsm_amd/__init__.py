@@ -198,6 +198,7 @@ def lib():
         "mdsm_plan_upload": [vp],
         "mdsm_probe_lds_read": [u32, u32, C.POINTER(f64)],
         "mdsm_probe_lds_write": [u32, u32, C.POINTER(f64)],
+        "mdsm_probe_lds_read_form": [u32, u32, C.POINTER(f64)],
         "mdsm_dedisperse_noalloc": [vp, vp, i64, vp, vp],
         "mdsm_plan_get_tuning": [vp, C.POINTER(Tuning)],
         "mdsm_plan_autotune": [vp, vp, i64, vp, vp, vp, u32, C.POINTER(Tuning)],
@@ -701,6 +702,15 @@ def probe_lds_read(bytes_per_lane: int, waves_per_simd: int) -> float:
     """Measured LDS read bandwidth (TB/s) of the current GPU."""
     v = C.c_double()
     check(lib().mdsm_probe_lds_read(int(bytes_per_lane), int(waves_per_simd), C.byref(v)))
+    return v.value
+
+
+def probe_lds_read_form(form: int, waves_per_simd: int) -> float:
+    """Measured LDS read bandwidth (TB/s) of read form 1..4 (ds_read2st64_b32,
+    ds_read2st64_b64, ds_read_b64 at a 4-byte-aligned address, ds_read2_b32
+    of adjacent dwords)."""
+    v = C.c_double()
+    check(lib().mdsm_probe_lds_read_form(int(form), int(waves_per_simd), C.byref(v)))
     return v.value
 
 

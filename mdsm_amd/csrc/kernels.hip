@@ -204,12 +204,12 @@ __device__ __forceinline__ void bfly_half(double *v, int lane, int m)
 //    x[2j+p] | x[2j+p+1] << 16; offset o reads dwords (o>>1) + L and
 //    (o>>1) + L + 64 of plane o&1 (one ds_read2st64_b32): lane L holds
 //    samples 2L, 2L+1, 128+2L, 129+2L.  Half the staged bytes of four phases;
-//    the reads run slower (ds_read2st64_b32 74 TB/s against 85 for
+//    the reads run slower (ds_read2st64_b32 75 TB/s against 94 for
 //    ds_read_b64 at 4 waves per SIMD), which the integer rows, bound by their
 //    reads, do not repay (the three-range uint8 plan of 1024 channels x 2^20
 //    samples 46.9 -> 56.0 ms) and float16 rows, bound by their
 //    v_fma_mix_f32, do.  (An 8-byte ds_read_b64 at a 4-byte-aligned address
-//    runs 16x slower, ds_read2_b32 of adjacent dwords at half the rate.)
+//    runs 19x slower, ds_read2_b32 of adjacent dwords at 40% of the rate.)
 //  * float32: two phases of float2 entries, entry j of phase p =
 //    (x[2j+p], x[2j+p+1]), read like the float16 planes (ds_read2st64_b64).
 //    (Four phases of float4 entries read by ds_read_b128 read faster, but

@@ -65,7 +65,7 @@ def _serve(steps: int, warmup: int) -> None:
 
 def _lds_probe() -> None:
     """LDS read and write bandwidth of ds_read/ds_write _b32/_b64/_b128 at 1,
-    2, 4 and 8 waves per SIMD."""
+    2, 4 and 8 waves per SIMD, then the other read forms at 4 and 8."""
     import mdsm_amd as M
     print("| bytes per lane | waves per SIMD | read TB/s | write TB/s |")
     print("|---|---|---|---|")
@@ -74,6 +74,17 @@ def _lds_probe() -> None:
             r = max(M.probe_lds_read(w, occ) for _ in range(3))
             wr = max(M.probe_lds_write(w, occ) for _ in range(3))
             print(f"| {w} | {occ} | {r:.1f} | {wr:.1f} |", flush=True)
+    print()
+    print("| read form | bytes per lane | waves per SIMD | read TB/s |")
+    print("|---|---|---|---|")
+    forms = {1: ("ds_read2st64_b32, two dwords 256 B apart", 8),
+             2: ("ds_read2st64_b64, two 8-byte entries 512 B apart", 16),
+             3: ("ds_read_b64 at a 4-byte-aligned address", 8),
+             4: ("ds_read2_b32, adjacent dwords at a 4-byte-aligned address", 8)}
+    for f, (name, b) in forms.items():
+        for occ in (4, 8):
+            r = max(M.probe_lds_read_form(f, occ) for _ in range(3))
+            print(f"| {name} | {b} | {occ} | {r:.1f} |", flush=True)
 
 
 def _copy() -> None:
