@@ -226,3 +226,14 @@ def test_plan_lds_bytes_float16_two_planes():
         writes += ntiles * 4 * int(Wrow.sum())
     assert sh.shape[0] % 64 <= 32, "the last block should run 4 trials per wave here"
     assert p.lds_bytes(0) == (reads, writes)
+
+
+def test_lds_probes_reject_bad_arguments():
+    """The LDS probes validate their arguments before touching a device."""
+    for f in (lambda: M.probe_lds_read(3, 4), lambda: M.probe_lds_write(32, 4),
+              lambda: M.probe_lds_read(8, 0), lambda: M.probe_lds_read(8, 9),
+              lambda: M.probe_lds_read_form(0, 4), lambda: M.probe_lds_read_form(5, 4),
+              lambda: M.probe_lds_read_form(1, 9)):
+        with pytest.raises(M.MdsmError):
+            f()
+
