@@ -42,16 +42,39 @@ LDS_B64_AT_4_WAVES_TBS = 97.4
 LDS_WRITE_B64_AT_4_WAVES_TBS = 50.8
 
 
-def lds_roof(reads, writes, seconds):
+def probe_lds_rates(M):
+    """ds_read_b64 and ds_write_b64 rates (TB/s) at 4 waves per SIMD on this
+    GPU, best of three probe launches each (a few ms); None if the probe
+    fails."""
+    try:
+        return (max(M.probe_lds_read(8, 4) for _ in range(3)),
+                max(M.probe_lds_write(8, 4) for _ in range(3)))
+    except Exception as e:  # the roof then falls back to the profile's rates
+        print(f"[bench] LDS probe failed: {e}", file=sys.stderr, flush=True)
+        return None
+
+
+def lds_roof(reads, writes, seconds, live=None):
     """The dedispersion kernel's LDS roof at its occupancy (4 waves per SIMD):
     the trial reads at the ds_read_b64 rate plus the staging stores at the
-    ds_write_b64 rate, against the measured time."""
-    roof_s = reads / (LDS_B64_AT_4_WAVES_TBS * 1e12) + writes / (LDS_WRITE_B64_AT_4_WAVES_TBS * 1e12)
+    ds_write_b64 rate, against the measured time.  The rates are those
+    probed on this GPU in this run (live), else the profile's."""
+    def roof(rd, wr):
+        return reads / (rd * 1e12) + writes / (wr * 1e12)
+    prof_s = roof(LDS_B64_AT_4_WAVES_TBS, LDS_WRITE_B64_AT_4_WAVES_TBS)
+    rd, wr = live if live else (LDS_B64_AT_4_WAVES_TBS, LDS_WRITE_B64_AT_4_WAVES_TBS)
+    roof_s = roof(rd, wr)
     return {"read_bytes": reads, "write_bytes": writes, "roof_ms": roof_s * 1e3,
             "rw_frac": roof_s / seconds,
-            "rw_note": "LDS time of the trial reads (ds_read_b64, 97.4 TB/s) and staging stores "
-                       "(ds_write_b64, 50.8 TB/s) at 4 waves per SIMD, each at its probed rate "
-                       "(profiles/lds_probe.md), over the measured kernel time"}
+            "read_b64_tbs": rd, "write_b64_tbs": wr,
+            "rates_source": ("probed on this GPU in this run (mdsm_probe_lds_read/_write, 8 bytes "
+                             "per lane, 4 waves per SIMD, best of 3)") if live
+                            else "profiles/lds_probe.md",
+            "rw_frac_at_profile_rates": prof_s / seconds,
+            "rw_note": "LDS time of the trial reads (ds_read_b64) and staging stores "
+                       "(ds_write_b64) at 4 waves per SIMD, each at its probed rate, over the "
+                       "measured kernel time; rw_frac_at_profile_rates prices them at "
+                       "profiles/lds_probe.md's 97.4 and 50.8 TB/s"}
 # BASELINE's headline metric: throughput, its fraction of the HBM roofline,
 # real-time factor and p99 block latency, against the plain float32 kernel
 BASELINE_METRIC = ("DM\u00b7chan\u00b7samples/s, %HBM roofline, RT factor, p99 block latency "
@@ -204,7 +227,7 @@ def serve(M, w, args):
     tr = read_traffic(prof)
     lds_bytes_blk = units_blk * 2.0
     occ = {"occupancy_theoretical": tp.occupancy(0)}
-    lds_rw = lds_roof(*tp.lds_bytes(0), ded_s_blk)
+    lds_rw = lds_roof(*tp.lds_bytes(0), ded_s_blk, probe_lds_rates(M))
     # latency at the real arrival rate (host pushes: PCIe included)
     print("[bench] cfg5: latency at the real arrival rate", file=sys.stderr, flush=True)
 
@@ -307,8 +330,7 @@ def serve(M, w, args):
         "lds": {"bytes_per_block": lds_bytes_blk,
                 "achieved_tbs": lds_bytes_blk / ded_s_blk / 1e12, "peak_tbs": LDS_PEAK_TBS,
                 "frac": lds_bytes_blk / ded_s_blk / 1e12 / LDS_PEAK_TBS,
-                "peak_source": LDS_PEAK_SOURCE, "b64_at_4_waves_tbs": LDS_B64_AT_4_WAVES_TBS,
-                **occ, **lds_rw},
+                "peak_source": LDS_PEAK_SOURCE, **occ, **lds_rw},
         "kernels_ms_per_block": {k: v[0] / nblk for k, v in kern.items()},
         "mean_batch": mean_batch_thr, "pulse_blocks": pulses_thr, "tuning": tuning,
         "server_stats": {k: int(v) for k, v in st.items()},
@@ -884,7 +906,7 @@ def main():
                 # float32 rows read through ds_read2st64_b64 at another rate
                 **(lds_roof(*[sum(v) for v in zip(*[plan.lds_bytes(i) for i in
                                                      range(2 if info.is_subband else 1)])],
-                            avg_launch_s) if w["dtype"] == "u8" else {})},
+                            avg_launch_s, probe_lds_rates(M)) if w["dtype"] == "u8" else {})},
         "valu": {"adds_per_launch": adds_per_launch,
                  "achieved_tadds": adds_per_launch / avg_launch_s / 1e12,
                  "peak_tadds": VALU_ADD_PEAK_T,
