@@ -250,3 +250,85 @@ def test_zero_copy_acquire_commit_matches_push(gpu):
     for key in outs[0]:
         assert np.array_equal(outs[0][key][0], outs[1][key][0])
         assert _key(outs[0][key][1]) == _key(outs[1][key][1])
+
+
+def test_float16_beam_in_a_mixed_batch(gpu):
+    """A float16 beam (no clipper: it runs on uint8 beams) served beside a
+    uint8 beam, their first blocks admitted as one batch (one launch of the
+    kernel built for every dtype, row statistics on): each beam's outputs and
+    candidates equal that beam served alone; the float16 outputs are within
+    the path's 1e-5 normwise tolerance of the FP64 oracle over the
+    concatenated stream; its candidates are the oracle's streaming search of
+    the served series (the FP64 tree on the same float32 values; the row
+    statistics' FP64 sums may differ in rounding order, so S/N agree to 1e-6
+    and the sets agree away from a 1e-6 band around the threshold)."""
+    from mdsm_amd.server import Server
+    from conftest import rel_err
+    M = gpu
+    C, N, nblocks = 64, 4096, 3
+    ranges = ((0.0, 2.0, 40, 1), (80.0, 4.0, 20, 2))
+    print("case server float16 beam in a mixed batch", flush=True)
+    kw16 = dict(_beam_kwargs(C, N, ranges, clip=0.0), dtype="f16")
+    kw8 = _beam_kwargs(C, N, ranges)
+    x16 = M.synth_filterbank(C, N * nblocks, "f16", seed=311, gain=(0.5, 2.0), fch1=1500.0,
+                             foff=-300.0 / C, pulses=[(30.0, 2500, 4, 3.0), (50.0, N + 1000, 8, 2.0),
+                                                     (20.0, 2 * N - 200, 16, 2.0)])
+    x8 = _stream(M, C, N, nblocks, 312, [(40.0, 3000, 4, 3.0)])
+    alone = {}
+    for name, kw, x in (("f16", kw16, x16), ("u8", kw8, x8)):
+        srv = Server(max_batch=1)
+        b = srv.open_beam(**kw)
+        r = _run(srv, [(b, x)], N, nblocks)
+        alone[name] = {k: r[(b.id, k)] for k in range(nblocks)}
+        srv.close()
+    srv = Server(max_batch=4)
+    b16 = srv.open_beam(**kw16)
+    b8 = srv.open_beam(**kw8)
+    Mo = b16.overlap
+    mix = _run(srv, [(b16, x16), (b8, x8)], N, nblocks, hold_first=True)
+    srv.close()
+    for k in range(nblocks):
+        for name, b in (("f16", b16), ("u8", b8)):
+            o_a, c_a, _ = alone[name][k]
+            o_m, c_m, _ = mix[(b.id, k)]
+            assert np.array_equal(o_a, o_m), f"{name} block {k}: output differs in the mix"
+            assert _key(c_a) == _key(c_m), f"{name} block {k}: candidates differ in the mix"
+    full = np.concatenate([np.zeros((C, Mo), np.float16), x16], axis=1)
+    series = [[] for _ in ranges]
+    for k in range(nblocks):
+        seg = np.ascontiguousarray(full[:, k * N:k * N + Mo + N])
+        got = alone["f16"][k][0]
+        off = 0
+        for r, (lo, step, nd, kk) in enumerate(ranges):
+            ts = np.float32(64e-6) * np.float32(kk)
+            sh = oracle.shift_table_ref(1500.0, -300.0 / C, C, lo, step, nd, ts)
+            nout = (Mo + N) // kk - int(sh.max())
+            g = got[off:off + nd * nout].reshape(nd, nout)
+            off += nd * nout
+            e, a = rel_err(g, oracle.range_f64(seg, kk, sh, nout))
+            print(f"block {k} range {r}: rel {e:.3e} maxabs {a:.3e}")
+            assert e <= 1e-5
+            series[r].append(g[:, :N // kk])
+    thr = 6.0
+    first = 0
+    nfound = 0
+    for r, (lo, step, nd, kk) in enumerate(ranges):
+        ref = oracle.boxcar_stream(np.concatenate(series[r], axis=1), N // kk, list(WIDTHS), thr)
+        for k in range(nblocks):
+            c = alone["f16"][k][1]
+            sel = (c["dm"] >= first) & (c["dm"] < first + nd)
+            gotd = {(int(d) - first, int(t), int(w)): float(s)
+                    for d, t, w, s in zip(c["dm"][sel], c["t"][sel], c["width"][sel], c["snr"][sel])}
+            rk = ref[ref["blk"] == k]
+            expd = {(int(d), int(t), int(w)): float(s)
+                    for d, t, w, s in zip(rk["dm"], rk["t"], rk["width"], rk["snr"])}
+            for key in set(gotd) | set(expd):
+                if key in gotd and key in expd:
+                    assert abs(gotd[key] - expd[key]) <= 1e-6 * abs(expd[key]), key
+                else:
+                    s = gotd.get(key, expd.get(key))
+                    assert abs(s - thr) <= 1e-6 * thr, f"range {r} block {k}: {key} {s}"
+            nfound += len(expd)
+        first += nd
+    print(f"float16 beam: {nfound} candidates")
+    assert nfound > 0
