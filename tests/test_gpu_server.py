@@ -252,11 +252,12 @@ def test_zero_copy_acquire_commit_matches_push(gpu):
         assert _key(outs[0][key][1]) == _key(outs[1][key][1])
 
 
-def test_float16_beam_in_a_mixed_batch(gpu):
-    """A float16 beam (no clipper: it runs on uint8 beams) served beside a
+@pytest.mark.parametrize("fdt", ["f16", "f32"])
+def test_float_beam_in_a_mixed_batch(gpu, fdt):
+    """A float16 or float32 beam (no clipper: it runs on uint8 beams) served beside a
     uint8 beam, their first blocks admitted as one batch (one launch of the
     kernel built for every dtype, row statistics on): each beam's outputs and
-    candidates equal that beam served alone; the float16 outputs are within
+    candidates equal that beam served alone; the float outputs are within
     the path's 1e-5 normwise tolerance of the FP64 oracle over the
     concatenated stream; its candidates are the oracle's streaming search of
     the served series (the FP64 tree on the same float32 values; the row
@@ -267,10 +268,10 @@ def test_float16_beam_in_a_mixed_batch(gpu):
     M = gpu
     C, N, nblocks = 64, 4096, 3
     ranges = ((0.0, 2.0, 40, 1), (80.0, 4.0, 20, 2))
-    print("case server float16 beam in a mixed batch", flush=True)
-    kw16 = dict(_beam_kwargs(C, N, ranges, clip=0.0), dtype="f16")
+    print(f"case server {fdt} beam in a mixed batch", flush=True)
+    kw16 = dict(_beam_kwargs(C, N, ranges, clip=0.0), dtype=fdt)
     kw8 = _beam_kwargs(C, N, ranges)
-    x16 = M.synth_filterbank(C, N * nblocks, "f16", seed=311, gain=(0.5, 2.0), fch1=1500.0,
+    x16 = M.synth_filterbank(C, N * nblocks, fdt, seed=311, gain=(0.5, 2.0), fch1=1500.0,
                              foff=-300.0 / C, pulses=[(30.0, 2500, 4, 3.0), (50.0, N + 1000, 8, 2.0),
                                                      (20.0, 2 * N - 200, 16, 2.0)])
     x8 = _stream(M, C, N, nblocks, 312, [(40.0, 3000, 4, 3.0)])
@@ -293,7 +294,7 @@ def test_float16_beam_in_a_mixed_batch(gpu):
             o_m, c_m, _ = mix[(b.id, k)]
             assert np.array_equal(o_a, o_m), f"{name} block {k}: output differs in the mix"
             assert _key(c_a) == _key(c_m), f"{name} block {k}: candidates differ in the mix"
-    full = np.concatenate([np.zeros((C, Mo), np.float16), x16], axis=1)
+    full = np.concatenate([np.zeros((C, Mo), x16.dtype), x16], axis=1)
     series = [[] for _ in ranges]
     for k in range(nblocks):
         seg = np.ascontiguousarray(full[:, k * N:k * N + Mo + N])
@@ -330,5 +331,5 @@ def test_float16_beam_in_a_mixed_batch(gpu):
                     assert abs(s - thr) <= 1e-6 * thr, f"range {r} block {k}: {key} {s}"
             nfound += len(expd)
         first += nd
-    print(f"float16 beam: {nfound} candidates")
+    print(f"{fdt} beam: {nfound} candidates")
     assert nfound > 0
