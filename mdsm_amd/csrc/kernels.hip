@@ -777,6 +777,60 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
             // place per pair.
             int v0 = vo[0], v1 = vo[1], v2 = vo[2], v3 = vo[3];
             const int np = cc >> 1;
+            if constexpr (kInt) {
+                // Pipelined by half pairs: the next pair's reads of trials
+                // 0..H-1 go into the registers the current pair's first-half
+                // adds have just freed, so the LDS queue does not drain
+                // between pairs; the data registers stay 2R.
+                constexpr int H = R / 2;
+                uint2 x[R], y[R];
+                auto issue = [&](int v, int r0) {
+#pragma unroll
+                    for (int r = r0; r < r0 + H; ++r) {
+                        x[r] = ld16<DT>(lane_base + bcast16(v, r));
+                        y[r] = ld16<DT>(lane_base + bcast16(v, 8 + r));
+                    }
+                };
+                auto add = [&](int r0) {
+#pragma unroll
+                    for (int r = r0; r < r0 + H; ++r) {
+                        ae[r] = ae[r] + x[r].x + y[r].x;
+                        ao[r] = ao[r] + x[r].y + y[r].y;
+                    }
+                };
+                if (np > 0) {
+                    __builtin_amdgcn_sched_barrier(0);
+                    issue(v0, 0);
+                    issue(v0, H);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+#pragma unroll 1
+                for (int pr = 0; pr + 1 < np; ++pr) {
+                    add(0);
+                    __builtin_amdgcn_sched_barrier(0);
+                    issue(v1, 0);
+                    __builtin_amdgcn_sched_barrier(0);
+                    add(H);
+                    __builtin_amdgcn_sched_barrier(0);
+                    issue(v1, H);
+                    __builtin_amdgcn_sched_barrier(0);
+                    v0 = v1;
+                    v1 = v2;
+                    v2 = v3;
+                }
+                if (np > 0) {
+                    add(0);
+                    add(H);
+                    v0 = v1;
+                }
+                if (cc & 1) {  // the chunk's last row on its own
+                    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                    for (int r = 0; r < R; ++r)
+                        add_entry(r, ld16<DT>(lane_base + bcast16(v0, r)));
+                }
+                return;
+            }
 #pragma unroll 1
             for (int pr = 0; pr < np; ++pr) {
                 // one pair's reads at a time: hoisting the next pair's reads
