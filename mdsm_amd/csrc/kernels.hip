@@ -267,6 +267,10 @@ struct Layout {
     static constexpr bool split = DT == DT_F32 || planes;
 };
 
+// Parts of a channel pair's reads in the pipelined trial loop (halves;
+// quarters measured 0.5-0.8% slower on the integer launches).
+constexpr int kPipe = 2;
+
 // Tile sample of a lane's accumulator q (0..3).
 template <int DT>
 __device__ __forceinline__ int lane_sample(int lane, int q)
@@ -770,85 +774,32 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 row(vo[pr], 1);
             }
         } else {
-            // A rolled loop over the chunk's full pairs: the accumulators are
+            // A rolled loop over the chunk's full pairs (integer rows, and
+            // float16 rows on per-lane offsets): the accumulators are
             // loop-carried and updated in place.  (Unrolled with an exit per
             // pair count, the exits' merges cost 16 register copies per pair.)
             // The pair's offset register is vo[0]; the others move down one
             // place per pair.
             int v0 = vo[0], v1 = vo[1], v2 = vo[2], v3 = vo[3];
             const int np = cc >> 1;
-            if constexpr (kInt) {
-                // Pipelined by half pairs: the next pair's reads of trials
-                // 0..H-1 go into the registers the current pair's first-half
-                // adds have just freed, so the LDS queue does not drain
-                // between pairs; the data registers stay 2R.
-                constexpr int H = R / 2;
-                uint2 x[R], y[R];
-                auto issue = [&](int v, int r0) {
+            // Pipelined by parts of a pair (kPipe parts of R/kPipe trials):
+            // the next pair's reads of a part go into the registers the
+            // current pair's adds of that part have just freed, so the
+            // LDS queue does not drain between pairs; the data registers
+            // stay 2R.
+            constexpr int G = kPipe, H = R / kPipe;
+            static_assert(R % kPipe == 0, "parts of whole trials");
+            uint2 x[R], y[R];
+            auto issue = [&](int v, int r0) {
 #pragma unroll
-                    for (int r = r0; r < r0 + H; ++r) {
-                        x[r] = ld16<DT>(lane_base + bcast16(v, r));
-                        y[r] = ld16<DT>(lane_base + bcast16(v, 8 + r));
-                    }
-                };
-                auto add = [&](int r0) {
+                for (int r = r0; r < r0 + H; ++r) {
+                    x[r] = ld16<DT>(lane_base + bcast16(v, r));
+                    y[r] = ld16<DT>(lane_base + bcast16(v, 8 + r));
+                }
+            };
+            auto add = [&](int r0) {
 #pragma unroll
-                    for (int r = r0; r < r0 + H; ++r) {
-                        ae[r] = ae[r] + x[r].x + y[r].x;
-                        ao[r] = ao[r] + x[r].y + y[r].y;
-                    }
-                };
-                if (np > 0) {
-                    __builtin_amdgcn_sched_barrier(0);
-                    issue(v0, 0);
-                    issue(v0, H);
-                    __builtin_amdgcn_sched_barrier(0);
-                }
-#pragma unroll 1
-                for (int pr = 0; pr + 1 < np; ++pr) {
-                    add(0);
-                    __builtin_amdgcn_sched_barrier(0);
-                    issue(v1, 0);
-                    __builtin_amdgcn_sched_barrier(0);
-                    add(H);
-                    __builtin_amdgcn_sched_barrier(0);
-                    issue(v1, H);
-                    __builtin_amdgcn_sched_barrier(0);
-                    v0 = v1;
-                    v1 = v2;
-                    v2 = v3;
-                }
-                if (np > 0) {
-                    add(0);
-                    add(H);
-                    v0 = v1;
-                }
-                if (cc & 1) {  // the chunk's last row on its own
-                    __builtin_amdgcn_sched_barrier(0);
-#pragma unroll
-                    for (int r = 0; r < R; ++r)
-                        add_entry(r, ld16<DT>(lane_base + bcast16(v0, r)));
-                }
-                return;
-            }
-#pragma unroll 1
-            for (int pr = 0; pr < np; ++pr) {
-                // one pair's reads at a time: hoisting the next pair's reads
-                // above these adds would need twice the data registers
-                __builtin_amdgcn_sched_barrier(0);
-                uint2 x[R], y[R];
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    x[r] = ld16<DT>(lane_base + bcast16(v0, r));
-                    y[r] = ld16<DT>(lane_base + bcast16(v0, 8 + r));
-                }
-                // all 16 reads in flight before the first add waits on one
-                __builtin_amdgcn_sched_barrier(0);
-                v0 = v1;
-                v1 = v2;
-                v2 = v3;
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
+                for (int r = r0; r < r0 + H; ++r) {
                     if constexpr (kInt) {
                         ae[r] = ae[r] + x[r].x + y[r].x;
                         ao[r] = ao[r] + x[r].y + y[r].y;
@@ -857,6 +808,30 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                         add_entry(r, y[r]);
                     }
                 }
+            };
+            if (np > 0) {
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int g = 0; g < G; ++g) issue(v0, g * H);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll 1
+            for (int pr = 0; pr + 1 < np; ++pr) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    add(g * H);
+                    __builtin_amdgcn_sched_barrier(0);
+                    issue(v1, g * H);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                v0 = v1;
+                v1 = v2;
+                v2 = v3;
+            }
+            if (np > 0) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) add(g * H);
+                v0 = v1;
             }
             if (cc & 1) {  // the chunk's last row on its own
                 __builtin_amdgcn_sched_barrier(0);
