@@ -751,31 +751,10 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
     };
     // sum rows [0, cc) of the staged chunk into the accumulators
     auto sum_chunk = [&](const char *lane_base, int cc) {
-        if constexpr (DT == DT_F32) {
-            auto row = [&](int o_pair, int half) {
-                float4 x[R];
-#pragma unroll
-                for (int r = 0; r < R; ++r) x[r] = ld_f32(lane_base + bcast16(o_pair, 8 * half + r));
-#pragma unroll
-                for (int r = 0; r < R; ++r) {
-                    f[r][0] += x[r].x;
-                    f[r][1] += x[r].y;
-                    f[r][2] += x[r].z;
-                    f[r][3] += x[r].w;
-                }
-            };
-#pragma unroll
-            for (int pr = 0; pr < 4; ++pr) {
-                if (2 * pr >= cc) break;
-                __builtin_amdgcn_sched_barrier(0);
-                row(vo[pr], 0);
-                if (2 * pr + 1 >= cc) break;
-                __builtin_amdgcn_sched_barrier(0);
-                row(vo[pr], 1);
-            }
-        } else {
-            // A rolled loop over the chunk's full pairs (integer rows, and
-            // float16 rows on per-lane offsets): the accumulators are
+        {
+            // A rolled loop over the chunk's full pairs (integer rows; float32
+            // rows take the scalar offsets, sum_chunk_s, and float16 rows on
+            // per-lane offsets sum_chunk_staged below): the accumulators are
             // loop-carried and updated in place.  (Unrolled with an exit per
             // pair count, the exits' merges cost 16 register copies per pair.)
             // The pair's offset register is vo[0]; the others move down one
@@ -831,6 +810,88 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
             if (np > 0) {
 #pragma unroll
                 for (int g = 0; g < G; ++g) add(g * H);
+                v0 = v1;
+            }
+            if (cc & 1) {  // the chunk's last row on its own
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int r = 0; r < R; ++r)
+                    add_entry(r, ld16<DT>(lane_base + bcast16(v0, r)));
+            }
+        }
+    };
+
+    // The same loop for float16 rows on per-lane offsets, with the next
+    // chunk's staging (the other buffer) run between the last pair's first-
+    // and second-half adds, its stores queued behind that pair's reads (the
+    // two-stage subband plan of 4096 float16 channels 18.0 -> 17.3 ms; for
+    // integer rows, bound by their reads, it cost 1.5%).  A lambda of its
+    // own: a template parameter on sum_chunk changes the register allocation
+    // of the integer instances.
+    auto sum_chunk_staged = [&](const char *lane_base, int cc, auto &&stage) {
+        {
+            // A rolled loop over the chunk's full pairs (integer rows; float32
+            // rows take the scalar offsets, sum_chunk_s, and float16 rows on
+            // per-lane offsets sum_chunk_staged below): the accumulators are
+            // loop-carried and updated in place.  (Unrolled with an exit per
+            // pair count, the exits' merges cost 16 register copies per pair.)
+            // The pair's offset register is vo[0]; the others move down one
+            // place per pair.
+            int v0 = vo[0], v1 = vo[1], v2 = vo[2], v3 = vo[3];
+            const int np = cc >> 1;
+            // Pipelined by parts of a pair (kPipe parts of R/kPipe trials):
+            // the next pair's reads of a part go into the registers the
+            // current pair's adds of that part have just freed, so the
+            // LDS queue does not drain between pairs; the data registers
+            // stay 2R.
+            constexpr int G = kPipe, H = R / kPipe;
+            static_assert(R % kPipe == 0, "parts of whole trials");
+            uint2 x[R], y[R];
+            auto issue = [&](int v, int r0) {
+#pragma unroll
+                for (int r = r0; r < r0 + H; ++r) {
+                    x[r] = ld16<DT>(lane_base + bcast16(v, r));
+                    y[r] = ld16<DT>(lane_base + bcast16(v, 8 + r));
+                }
+            };
+            auto add = [&](int r0) {
+#pragma unroll
+                for (int r = r0; r < r0 + H; ++r) {
+                    if constexpr (kInt) {
+                        ae[r] = ae[r] + x[r].x + y[r].x;
+                        ao[r] = ao[r] + x[r].y + y[r].y;
+                    } else {
+                        add_entry(r, x[r]);
+                        add_entry(r, y[r]);
+                    }
+                }
+            };
+            if (np > 0) {
+                __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+                for (int g = 0; g < G; ++g) issue(v0, g * H);
+                __builtin_amdgcn_sched_barrier(0);
+            }
+#pragma unroll 1
+            for (int pr = 0; pr + 1 < np; ++pr) {
+#pragma unroll
+                for (int g = 0; g < G; ++g) {
+                    add(g * H);
+                    __builtin_amdgcn_sched_barrier(0);
+                    issue(v1, g * H);
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                v0 = v1;
+                v1 = v2;
+                v2 = v3;
+            }
+            if (np > 0) add(0);
+            __builtin_amdgcn_sched_barrier(0);
+            stage();
+            __builtin_amdgcn_sched_barrier(0);
+            if (np > 0) {
+#pragma unroll
+                for (int g = 1; g < G; ++g) add(g * H);
                 v0 = v1;
             }
             if (cc & 1) {  // the chunk's last row on its own
@@ -901,6 +962,20 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 flush();
                 acc_rows = 0;
             }
+        }
+        if constexpr (kVOff && DT == DT_F16) {
+            sum_chunk_staged(lds + (ch & 1) * bufsz + lane * kEB, cc, [&] {
+                if (more) {
+                    if (ch + 2 < nch)
+                        load_colbase<true>(cb, ab + 2 * (c1 + ccr), min(ccr, bd.nrows - c1 - ccr),
+                                           wave);
+                    load_offsets(vo, c1);
+                    store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, cs, cc1, wave, lane);
+                }
+            });
+            acc_rows += cc;
+            __syncthreads();
+            continue;
         }
         if constexpr (kVOff)
             sum_chunk(lds + (ch & 1) * bufsz + lane * kEB, cc);
