@@ -15,6 +15,7 @@
 
 #include <algorithm>
 #include <atomic>
+#include <type_traits>
 
 #include "mdsm_internal.h"
 
@@ -903,6 +904,69 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         }
     };
 
+    // float32 rows on per-lane offsets (the kernel built for float data
+    // alone): one row of R float4 reads at a time, pipelined by half rows --
+    // the next row's reads of trials 0..R/2-1 go into the registers the
+    // current row's adds of those trials have just freed; row i is half i&1
+    // of offset register i>>1.  The next chunk's staging runs between the
+    // last row's halves.
+    auto sum_chunk_f32v = [&](const char *lane_base, int cc, auto &&stage) {
+        constexpr int H = R / 2;
+        int v0 = vo[0], v1 = vo[1], v2 = vo[2], v3 = vo[3];
+        const int np = cc >> 1;
+        float4 x[R];
+        auto issue = [&](int v, auto half, int r0) {
+#pragma unroll
+            for (int r = r0; r < r0 + H; ++r)
+                x[r] = ld_f32(lane_base + bcast16(v, 8 * decltype(half)::value + r));
+        };
+        auto add = [&](int r0) {
+#pragma unroll
+            for (int r = r0; r < r0 + H; ++r) {
+                f[r][0] += x[r].x;
+                f[r][1] += x[r].y;
+                f[r][2] += x[r].z;
+                f[r][3] += x[r].w;
+            }
+        };
+        using Lo = std::integral_constant<int, 0>;
+        using Hi = std::integral_constant<int, 1>;
+        // add the row in flight while the next one (register v, half) is issued
+        auto step = [&](int v, auto half) {
+            add(0);
+            __builtin_amdgcn_sched_barrier(0);
+            issue(v, half, 0);
+            __builtin_amdgcn_sched_barrier(0);
+            add(H);
+            __builtin_amdgcn_sched_barrier(0);
+            issue(v, half, H);
+            __builtin_amdgcn_sched_barrier(0);
+        };
+        if (cc > 0) {
+            __builtin_amdgcn_sched_barrier(0);
+            issue(v0, Lo{}, 0);
+            issue(v0, Lo{}, H);
+            __builtin_amdgcn_sched_barrier(0);
+        }
+#pragma unroll 1
+        for (int pr = 0; pr + 1 < np; ++pr) {
+            step(v0, Hi{});  // row 2pr added, row 2pr+1 issued
+            step(v1, Lo{});  // row 2pr+1 added, row 2pr+2 issued
+            v0 = v1;
+            v1 = v2;
+            v2 = v3;
+        }
+        if (np > 0) {
+            step(v0, Hi{});
+            if (cc & 1) step(v1, Lo{});  // the odd last row
+        }
+        if (cc > 0) add(0);
+        __builtin_amdgcn_sched_barrier(0);
+        stage();
+        __builtin_amdgcn_sched_barrier(0);
+        if (cc > 0) add(H);
+    };
+
     // integer rows (uint8, uint16 decimated sums) take the per-lane offset
     // registers, and so do float16 rows in the kernel built for float data
     // alone; float32 rows, and float16 rows in the kernel of all dtypes (whose
@@ -913,7 +977,7 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
     // all dtypes cost the served 1024-channel uint8 block 0.7%, in the float
     // kernel they save the two-stage subband plan of 4096 float16 channels
     // 0.8%.)
-    constexpr bool kVOff = DT == DT_U8 || DT == DT_U16 || (DT == DT_F16 && kFloatCls);
+    constexpr bool kVOff = DT == DT_U8 || DT == DT_U16 || ((DT == DT_F16 || DT == DT_F32) && kFloatCls);
     int acc_rows = 0;
     PreRegs<DT> pre;
     if constexpr (kVOff)
@@ -963,8 +1027,8 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                 acc_rows = 0;
             }
         }
-        if constexpr (kVOff && DT == DT_F16) {
-            sum_chunk_staged(lds + (ch & 1) * bufsz + lane * kEB, cc, [&] {
+        if constexpr (kVOff && (DT == DT_F16 || DT == DT_F32)) {
+            auto stage = [&] {
                 if (more) {
                     if (ch + 2 < nch)
                         load_colbase<true>(cb, ab + 2 * (c1 + ccr), min(ccr, bd.nrows - c1 - ccr),
@@ -972,7 +1036,11 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
                     load_offsets(vo, c1);
                     store_chunk<DT>(pre, lds + ((ch + 1) & 1) * bufsz, cs, cc1, wave, lane);
                 }
-            });
+            };
+            if constexpr (DT == DT_F16)
+                sum_chunk_staged(lds + (ch & 1) * bufsz + lane * kEB, cc, stage);
+            else
+                sum_chunk_f32v(lds + (ch & 1) * bufsz + lane * kEB, cc, stage);
             acc_rows += cc;
             __syncthreads();
             continue;
