@@ -908,8 +908,9 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
     // alone): one row of R float4 reads at a time, pipelined by half rows --
     // the next row's reads of trials 0..R/2-1 go into the registers the
     // current row's adds of those trials have just freed; row i is half i&1
-    // of offset register i>>1.  The next chunk's staging runs between the
-    // last row's halves.
+    // of offset register i>>1.  The next chunk's staging runs after the
+    // chunk (between the last row's halves, as for float16 rows, it measured
+    // 1.5-2.5% slower on the 256-channel float32 plan of 512 DMs).
     auto sum_chunk_f32v = [&](const char *lane_base, int cc, auto &&stage) {
         constexpr int H = R / 2;
         int v0 = vo[0], v1 = vo[1], v2 = vo[2], v3 = vo[3];
@@ -960,11 +961,12 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
             step(v0, Hi{});
             if (cc & 1) step(v1, Lo{});  // the odd last row
         }
-        if (cc > 0) add(0);
+        if (cc > 0) {
+            add(0);
+            add(H);
+        }
         __builtin_amdgcn_sched_barrier(0);
         stage();
-        __builtin_amdgcn_sched_barrier(0);
-        if (cc > 0) add(H);
     };
 
     // integer rows (uint8, uint16 decimated sums) take the per-lane offset
