@@ -969,17 +969,16 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
         stage();
     };
 
-    // integer rows (uint8, uint16 decimated sums) take the per-lane offset
-    // registers, and so do float16 rows in the kernel built for float data
-    // alone; float32 rows, and float16 rows in the kernel of all dtypes (whose
-    // register allocation the integer paths set), the scalar offsets.
-    // (uint16 rows on scalar offsets: the three-range uint8 plan of 1024
-    // channels x 2^20 samples 48.7 ms against 46.9 with the registers, which
-    // spill 4 VGPRs of the uint16 path; float16 rows on them in the kernel of
-    // all dtypes cost the served 1024-channel uint8 block 0.7%, in the float
-    // kernel they save the two-stage subband plan of 4096 float16 channels
-    // 0.8%.)
-    constexpr bool kVOff = DT == DT_U8 || DT == DT_U16 || ((DT == DT_F16 || DT == DT_F32) && kFloatCls);
+    // integer and float16 rows take the per-lane offset registers, and so do
+    // float32 rows in the kernel built for float data alone; float32 rows in
+    // the kernel of all dtypes (mixed batches), the scalar offsets.  (uint16
+    // rows on scalar offsets: the three-range uint8 plan of 1024 channels x
+    // 2^20 samples 48.7 ms against 46.9 with the registers.  Float16 rows on
+    // them in the kernel of all dtypes, with their own pipelined loop: the
+    // served 1024-channel uint8 block 4.67 -> 4.66 ms, the 64-beam batch
+    // 1.642 -> 1.631 s per step; before the pipelined loops they had cost the
+    // served block 0.7%.)
+    constexpr bool kVOff = DT == DT_U8 || DT == DT_U16 || DT == DT_F16 || (DT == DT_F32 && kFloatCls);
     int acc_rows = 0;
     PreRegs<DT> pre;
     if constexpr (kVOff)
