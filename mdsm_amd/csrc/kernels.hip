@@ -1155,6 +1155,12 @@ __device__ __forceinline__ void dedisperse_lds_t(const ItemCtx &a, const BlockDe
 // kernel of all dtypes (an integer-only kernel was about 1% slower than it on
 // the three-range uint8 plan of 1024 channels x 2^20 samples).
 constexpr int kClsInt = 1, kClsFloat = 2, kClsAll = 3;
+// Mixed launches (integer and float rows) without row statistics: the kernel
+// of all dtypes, with float32 rows on the per-lane offsets as in the float
+// kernel.  (The 64-beam batch, whose decimated float16 ranges are float32
+// rows: 1.630 -> 1.574 s per step.  The same change in the kernel that
+// integer-only and served launches take cost them 0.5-1.9%.)
+constexpr int kClsMixed = 7;
 
 template <int kCls, bool kStats>
 __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, int wave,
@@ -1187,7 +1193,9 @@ __device__ __forceinline__ void dedisperse_item(const DedispArgs &a, int64_t f, 
     ItemCtx ctx{uniform_ptr(sg.rel), uniform_ptr(sg.abase), uniform_ptr(jb.in),
                 uniform_ptr(jb.out), uniform_ptr(jb.work), jb.in_stride};
     const int t0 = tile_i * kTile;
-    constexpr bool kFloat = kCls == kClsFloat;  // the kernel built for float data alone
+    // float32 rows on per-lane offsets: the kernel built for float data alone,
+    // and the one for mixed launches
+    constexpr bool kFloat = kCls == kClsFloat || kCls == kClsMixed;
     // float blocks of at most kDB/2 trials run kRPW/2 trials per wave
     if constexpr ((kCls & kClsInt) != 0) {
         if (bd.dtype == DT_U8) {
@@ -1278,13 +1286,17 @@ hipError_t launch_dedisperse_spec(const LaunchSpec &L, hipStream_t s)
     const bool floats = (L.dtypes & ((1 << DT_F16) | (1 << DT_F32))) != 0;
     // launches with row statistics (the server's) take a kernel instance of
     // their own, so that the statistics epilogue costs the others nothing
-    const int cls = (floats && !ints) ? kClsFloat : (L.stats ? kClsAll + 1 : kClsAll);
-    void (*kern)(DedispArgs) = cls == kClsFloat ? dedisperse_kernel<kClsFloat, false>
-                               : cls == kClsAll ? dedisperse_kernel<kClsAll, false>
-                                                : dedisperse_kernel<kClsAll, true>;
+    const int cls = (floats && !ints) ? kClsFloat
+                    : L.stats         ? kClsAll + 1
+                    : floats          ? kClsMixed
+                                      : kClsAll;
+    void (*kern)(DedispArgs) = cls == kClsFloat   ? dedisperse_kernel<kClsFloat, false>
+                               : cls == kClsAll   ? dedisperse_kernel<kClsAll, false>
+                               : cls == kClsMixed ? dedisperse_kernel<kClsMixed, false>
+                                                  : dedisperse_kernel<kClsAll, true>;
     // the full gfx950 LDS to one workgroup, once per kernel (host threads may
     // launch concurrently; setting the attribute twice is harmless)
-    static std::atomic<bool> lds_attr[5];
+    static std::atomic<bool> lds_attr[8];
     if (!lds_attr[cls].load(std::memory_order_acquire)) {
         hipError_t e = hipFuncSetAttribute((const void *)kern,
                                            hipFuncAttributeMaxDynamicSharedMemorySize, kLdsMax);
